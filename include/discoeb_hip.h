@@ -1,0 +1,151 @@
+/*
+ * discoeb_hip.h — C ABI of the MI355X (gfx950) DISCO-EB perturbation hot path.
+ *
+ * The boundary replaces the reference's Python/JAX entry points
+ *   evolve_perturbations          src/discoeb/perturbations.py:926-997
+ *   evolve_perturbations_batched  src/discoeb/perturbations.py:1000-1061
+ *   evolve_one_mode               src/discoeb/perturbations.py:728-781
+ *   get_power                     src/discoeb/perturbations.py:1101-1123
+ * and, at the operator level, the stepper seam
+ *   Rodas5Transformed.step        src/discoeb/ode_integrators_stiff.py:718-834
+ *   model_synchronous (the vector field)  src/discoeb/perturbations.py:84-371
+ * The reference has no native FFI: its only seam is the Python call above.  The
+ * Python host package (disco-eb_amd/discoeb_amd) binds these functions with
+ * ctypes; INTEGRATION.md shows the binding.
+ *
+ * Conventions: plain pointers and sizes, row-major, FP64.  Pointers marked
+ * "device" are HIP device (or managed) memory owned by the caller; pointers
+ * marked "host" are read during the call only.  The library allocates no
+ * device memory and holds none between calls; scratch comes from the caller's
+ * workspace.  All work is enqueued on the caller's stream (hipStream_t passed
+ * as void*); no call synchronises the stream.  Every function returns 0 on
+ * success and a negative DEB_E_* code on invalid arguments or a HIP error;
+ * deb_last_error() gives a message (thread-local).
+ */
+#ifndef DISCOEB_HIP_H
+#define DISCOEB_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DEB_ABI_VERSION 1
+#define DEB_MAXNQ 16    /* maximum massive-neutrino momentum bins */
+#define DEB_NOUT_FIELDS 20
+
+/* Spline tables read on the path (param dict keys of the reference). */
+enum {
+  DEB_SPL_CS2A_LOGA = 0,     /* param['cs2a_of_loga_spline']      */
+  DEB_SPL_XE_LOGA = 1,       /* param['xe_of_loga_spline']        */
+  DEB_SPL_LOGRHONU_LOGA = 2, /* param['logrhonu_of_loga_spline']  */
+  DEB_SPL_LOGPNU_LOGA = 3,   /* param['logpnu_of_loga_spline']    */
+  DEB_SPL_TAU_A = 4,         /* param['tau_of_a_spline']          */
+  DEB_SPL_A_TAU = 5,         /* param['a_of_tau_spline']          */
+  DEB_SPL_XE_TAU = 6,        /* param['xe_of_tau_spline']         */
+  DEB_NSPLINE = 7
+};
+
+/* Per-mode status (the reference raises for the whole call; see throw=). */
+enum {
+  DEB_STATUS_OK = 0,
+  DEB_STATUS_MAX_STEPS = 1, /* diffrax max_steps reached before t1 */
+  DEB_STATUS_NONFINITE = 2, /* NaN/inf error norm                   */
+  DEB_STATUS_NOT_RUN = -1
+};
+
+enum {
+  DEB_E_ARG = -1,      /* invalid argument                        */
+  DEB_E_HIP = -2,      /* HIP runtime error                       */
+  DEB_E_UNSUPPORTED = -3 /* hierarchy sizes outside the kernel's layout */
+};
+
+enum { DEB_SOLVER_RODAS5 = 0 };
+
+/* Cosmology scalars (param dict keys of the reference, same meaning). */
+typedef struct {
+  double grhom, grhog, grhor;
+  double Omegam, Omegab, OmegaDE, Omegak;
+  double Neff, Nmnu, amnu;
+  double w_DE_0, w_DE_a, cs2_DE;
+  double YHe, H0, taumin;
+  double A_s, n_s, k_p;
+  double reserved[5];
+} deb_cosmo_t; /* 24 doubles */
+
+/* Hierarchy sizes and adaptive-solver settings (evolve_perturbations kwargs). */
+typedef struct {
+  int32_t lmaxg, lmaxgp, lmaxr, lmaxnu, nqmax;
+  int32_t max_steps;
+  int32_t return_full; /* 0: 20 output fields per output time; 1: raw state */
+  int32_t solver;      /* DEB_SOLVER_RODAS5 */
+  double rtol, atol;
+  double pcoeff, icoeff, dcoeff, factormax, factormin;
+} deb_config_t;
+
+/* ABI version (DEB_ABI_VERSION). */
+int deb_abi_version(void);
+
+/* Message for the last failing call on this host thread. */
+const char *deb_last_error(void);
+
+/* Number of state variables n (nvar of evolve_one_mode, perturbations.py:739). */
+int deb_nvar(const deb_config_t *cfg);
+
+/* Bytes of device workspace deb_solve_forward needs (caller allocates). */
+int deb_workspace_bytes(const deb_config_t *cfg, int n_cosmo, int nk, int nout, size_t *bytes);
+
+/*
+ * Forward solve of every (cosmology, k) mode: start time, adiabatic initial
+ * conditions, adaptive Rodas5 integration to max(tau_out) with outputs
+ * linearly interpolated at tau_out, conversion to the 20 output fields.
+ * Replaces evolve_perturbations (perturbations.py:926-997) for n_cosmo == 1
+ * and its multi-cosmology generalisation for n_cosmo > 1 (one persistent
+ * launch with a device work queue over all modes).
+ *
+ *   cosmo     device  [n_cosmo] deb_cosmo_t
+ *   tables    device  [n_cosmo][table_stride] doubles: for spline s the knots
+ *                     x, y, S (second derivatives) start at desc[2s] and each
+ *                     have desc[2s+1] entries
+ *   desc      host    [DEB_NSPLINE][2] int32 (offset, n) inside one cosmology
+ *   q, w      host    [nqmax] massive-nu momentum nodes / kernel weights
+ *                     (get_neutrino_momentum_bins, background.py:13-44)
+ *   kmodes    device  [n_cosmo][nk]  wavenumbers [1/Mpc]
+ *   tau_out   device  [n_cosmo][nout] increasing conformal output times
+ *   y_out     device  [n_cosmo][nk][nout][20 or nvar]
+ *   status    device  [n_cosmo][nk] int32 DEB_STATUS_*
+ *   nsteps    device  [n_cosmo][nk] int32 attempted steps (may be NULL)
+ *   workspace device  >= deb_workspace_bytes
+ */
+int deb_solve_forward(const deb_config_t *cfg, int n_cosmo, const deb_cosmo_t *cosmo, const double *tables,
+                      int64_t table_stride, const int32_t *desc, const double *q, const double *w, int nk,
+                      const double *kmodes, int nout, const double *tau_out, double *y_out, int32_t *status,
+                      int32_t *nsteps, void *workspace, size_t workspace_bytes, void *stream);
+
+/*
+ * Operator-level test hook: f = model_synchronous(tau, y) (perturbations.py:84-371)
+ * for m independent (cosmology index, k, tau, y) tuples, evaluated by the
+ * kernel's wave-parallel right-hand side.
+ *   ci [m] int32, kk [m], tau [m], y [m][nvar], f [m][nvar]: device.
+ */
+int deb_rhs(const deb_config_t *cfg, const deb_cosmo_t *cosmo, const double *tables, int64_t table_stride,
+            const int32_t *desc, const double *q, const double *w, int m, const int32_t *ci, const double *kk,
+            const double *tau, const double *y, double *f, void *stream);
+
+/*
+ * Operator-level test hook: one Rodas5Transformed step (ode_integrators_stiff.py:718-834)
+ * from (t0, y0) to t1 for m tuples, using the kernel's structured factorisation.
+ *   ci [m], kk [m], t0 [m], t1 [m], y0 [m][nvar], y1 [m][nvar], err [m][nvar]: device.
+ */
+int deb_rodas5_step(const deb_config_t *cfg, const deb_cosmo_t *cosmo, const double *tables, int64_t table_stride,
+                    const int32_t *desc, const double *q, const double *w, int m, const int32_t *ci,
+                    const double *kk, const double *t0, const double *t1, const double *y0, double *y1, double *err,
+                    void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DISCOEB_HIP_H */

@@ -1,0 +1,702 @@
+// DISCO-EB perturbation hot path: CPU restatement used as the ORACLE.
+//
+// TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+// cpu_baseline leg of bench.py may load this library; the product path never
+// does.  It restates, in plain FP64 C++, what the reference computes:
+//
+//   model_synchronous          src/discoeb/perturbations.py:84-371
+//   nu_perturb                 src/discoeb/perturbations.py:22-50
+//   get_aprimeoa               src/discoeb/background.py:100-122
+//   spline evaluate            src/discoeb/spline_interpolation.py:130-153
+//   convert_to_output_vars     src/discoeb/perturbations.py:374-523
+//   adiabatic_ics_one_mode     src/discoeb/perturbations.py:526-627
+//   determine_starting_time    src/discoeb/perturbations.py:630-681 (+ util.py:365-396)
+//   evolve_one_mode            src/discoeb/perturbations.py:728-781
+//   rms_norm_filtered          src/discoeb/perturbations.py:701-705
+//   Rodas5Transformed.step     src/discoeb/ode_integrators_stiff.py:622-834
+//   get_power                  src/discoeb/perturbations.py:1101-1123
+//   diffrax loop semantics     PIDController (p, i, d, factormin only on
+//                              rejection, safety 0.9, error order 5), dt0,
+//                              _clip_to_end (1e-10), SaveAt linear
+//                              interpolation, max_steps -> status.
+//
+// The Jacobian is exact (forward-mode duals, like jax.jacfwd) and the linear
+// algebra is a dense partial-pivot LU (like jax.scipy.linalg.lu_factor).
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "../include/discoeb_hip.h"
+
+namespace {
+
+struct Dual {
+  double v, d;
+  Dual(double v_ = 0.0, double d_ = 0.0) : v(v_), d(d_) {}
+};
+inline Dual operator+(Dual a, Dual b) { return {a.v + b.v, a.d + b.d}; }
+inline Dual operator-(Dual a, Dual b) { return {a.v - b.v, a.d - b.d}; }
+inline Dual operator-(Dual a) { return {-a.v, -a.d}; }
+inline Dual operator*(Dual a, Dual b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+inline Dual operator/(Dual a, Dual b) {
+  double q = a.v / b.v;
+  return {q, (a.d - q * b.d) / b.v};
+}
+inline Dual operator+(Dual a, double b) { return {a.v + b, a.d}; }
+inline Dual operator+(double b, Dual a) { return {a.v + b, a.d}; }
+inline Dual operator-(Dual a, double b) { return {a.v - b, a.d}; }
+inline Dual operator-(double b, Dual a) { return {b - a.v, -a.d}; }
+inline Dual operator*(Dual a, double b) { return {a.v * b, a.d * b}; }
+inline Dual operator*(double b, Dual a) { return {a.v * b, a.d * b}; }
+inline Dual operator/(Dual a, double b) { return {a.v / b, a.d / b}; }
+inline Dual operator/(double b, Dual a) {
+  double q = b / a.v;
+  return {q, -q * a.d / a.v};
+}
+inline Dual exp(Dual a) {
+  double e = std::exp(a.v);
+  return {e, e * a.d};
+}
+inline Dual log(Dual a) { return {std::log(a.v), a.d / a.v}; }
+inline Dual sqrt(Dual a) {
+  double s = std::sqrt(a.v);
+  return {s, 0.5 * a.d / s};
+}
+inline Dual pow(Dual a, double p) {
+  double v = std::pow(a.v, p);
+  return {v, p * std::pow(a.v, p - 1.0) * a.d};
+}
+inline double val(double x) { return x; }
+inline double val(Dual x) { return x.v; }
+using std::exp;
+using std::log;
+using std::pow;
+using std::sqrt;
+
+// ---------------------------------------------------------------------------
+// natural cubic spline evaluation: interval = clip(searchsorted_left(x, v)-1, 0, n-2)
+struct SplineView {
+  const double *x, *y, *S;
+  int n;
+};
+
+inline int interval(const SplineView &s, double v) {
+  // number of knots strictly below v (left-sided searchsorted)
+  int lo = 0, hi = s.n;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (s.x[mid] < v) lo = mid + 1; else hi = mid;
+  }
+  int i = lo - 1;
+  if (i < 0) i = 0;
+  if (i > s.n - 2) i = s.n - 2;
+  return i;
+}
+
+template <typename T>
+T spline_eval(const SplineView &s, T xv) {
+  int i = interval(s, val(xv));
+  double h = s.x[i + 1] - s.x[i];
+  T t = (xv - s.x[i]) / h;
+  T A = 1.0 - t;
+  T B = t;
+  return A * s.y[i] + B * s.y[i + 1] + ((A * A * A - A) * s.S[i] + (B * B * B - B) * s.S[i + 1]) * (h * h) / 6.0;
+}
+
+struct Model {
+  deb_cosmo_t c;
+  SplineView sp[DEB_NSPLINE];
+  double k;
+  int lmaxg, lmaxgp, lmaxr, lmaxnu, nq;
+  double q[DEB_MAXNQ], w[DEB_MAXNQ];
+  int nvar() const { return 12 + lmaxg + lmaxgp + lmaxr + nq * (lmaxnu + 1); }
+};
+
+template <typename T>
+T aprimeoa(const Model &m, T a) {
+  const deb_cosmo_t &c = m.c;
+  T rhonu = exp(spline_eval(m.sp[DEB_SPL_LOGRHONU_LOGA], log(a)));
+  T rhoQ = pow(a, -3.0 * (1.0 + c.w_DE_0 + c.w_DE_a)) * exp(3.0 * (a - 1.0) * c.w_DE_a);
+  T grho = c.grhom * c.Omegam / a + (c.grhog + c.grhor * (c.Neff + c.Nmnu * rhonu)) / (a * a) +
+           c.grhom * c.OmegaDE * rhoQ * a * a + c.grhom * c.Omegak;
+  return sqrt(grho / 3.0);
+}
+
+// model_synchronous: f = dy/dtau
+template <typename T>
+void rhs(const Model &m, T tau, const T *y, T *f) {
+  const deb_cosmo_t &c = m.c;
+  const int lg = m.lmaxg, lgp = m.lmaxgp, lr = m.lmaxr, lnu = m.lmaxnu, nq = m.nq;
+  const int n = m.nvar();
+  const double k = m.k, k2 = k * k;
+  const double Omegac = c.Omegam - c.Omegab;
+  const int iq0 = 10 + lg + lgp + lr;
+  const int iq1 = iq0 + nq, iq2 = iq1 + nq, iq3 = iq2 + nq;
+  T a = y[0];
+  T loga = log(a);
+  T eta = y[2], deltac = y[3], thetac = y[4], deltab = y[5], thetab = y[6];
+  T deltag = y[7], thetag = y[8], shearg = y[9] / 2.0;
+  T deltar = y[9 + lg + lgp], thetar = y[10 + lg + lgp], shearr = y[11 + lg + lgp] / 2.0;
+  (void)shearg;
+  (void)shearr;
+  T deltaq = y[n - 2], thetaq = y[n - 1];
+  T cs2 = spline_eval(m.sp[DEB_SPL_CS2A_LOGA], loga) / a;
+  T xe = spline_eval(m.sp[DEB_SPL_XE_LOGA], loga);
+  T photbar = c.grhog / (c.grhom * c.Omegab * a);
+  T pb43 = 4.0 / 3.0 * photbar;
+  double cs2_Q = c.cs2_DE;
+  T w_Q = c.w_DE_0 + c.w_DE_a * (1.0 - a);
+  T rho_Q = pow(a, -3.0 * (1.0 + c.w_DE_0 + c.w_DE_a)) * exp(3.0 * (a - 1.0) * c.w_DE_a);
+  T rppQ = (1.0 + w_Q) * rho_Q * c.grhom * c.OmegaDE * thetaq * a * a;
+  T aH = aprimeoa(m, a);
+  T w_Q_prime = -c.w_DE_a * aH * a;
+  T ca2_Q = w_Q - w_Q_prime / 3.0 / ((1.0 + w_Q) + 1e-6) / aH;
+  double akthom = 2.3038921003709498e-9 * (1.0 - c.YHe) * c.Omegab * c.H0 * c.H0;
+  T opac = xe * akthom / (a * a);
+  f[0] = aH * a;
+  // nu_perturb
+  T drhonu = 0.0, dpnu = 0.0, fnu = 0.0, shearnu = 0.0;
+  for (int i = 0; i < nq; ++i) {
+    T aq = a * c.amnu / m.q[i];
+    T v = 1.0 / sqrt(1.0 + aq * aq);
+    drhonu = drhonu + m.w[i] * y[iq0 + i] / v;
+    dpnu = dpnu + m.w[i] * y[iq0 + i] * v;
+    fnu = fnu + m.w[i] * y[iq1 + i];
+    shearnu = shearnu + m.w[i] * y[iq2 + i] * v;
+  }
+  dpnu = dpnu / 3.0;
+  shearnu = shearnu * 2.0 / 3.0;
+  T a2 = a * a;
+  T dgrho = c.grhom * (Omegac * deltac + c.Omegab * deltab) / a +
+            (c.grhog * deltag + c.grhor * (c.Neff * deltar + c.Nmnu * drhonu)) / a2 +
+            c.grhom * c.OmegaDE * deltaq * rho_Q * a2;
+  T dgpres = (c.grhog * deltag + c.grhor * c.Neff * deltar) / a2 / 3.0 + c.grhor * c.Nmnu * dpnu / a2 +
+             (cs2_Q * c.grhom * c.OmegaDE * deltaq * rho_Q * a2 + (cs2_Q - ca2_Q) * (3.0 * aH * rppQ / k2));
+  T dgtheta = c.grhom * (Omegac * thetac + c.Omegab * thetab) / a +
+              4.0 / 3.0 * (c.grhog * thetag + c.Neff * c.grhor * thetar) / a2 + c.Nmnu * c.grhor * k * fnu / a2 + rppQ;
+  f[1] = -(dgrho + 3.0 * dgpres) * a;
+  T hprime = (2.0 * k2 * eta + dgrho) / aH;
+  T etaprime = 0.5 * dgtheta / k2;
+  T alpha = (hprime + 6.0 * etaprime) / 2.0 / k2;
+  f[2] = etaprime;
+  f[3] = -thetac - 0.5 * hprime;
+  f[4] = -aH * thetac;
+  f[5] = -thetab - 0.5 * hprime;
+  f[6] = -aH * thetab + k2 * cs2 * deltab + pb43 * opac * (thetag - thetab);
+  const int ig = 7, igp = 8 + lg;
+  T polter = y[ig + 2] + y[igp + 0] + y[igp + 2];
+  f[ig + 0] = 4.0 / 3.0 * (-thetag - 0.5 * hprime);
+  f[ig + 1] = k2 * (0.25 * deltag - y[ig + 2] / 2.0) - opac * (thetag - thetab);
+  f[ig + 2] = 8.0 / 15.0 * (thetag + k2 * alpha) - 3.0 / 5.0 * k * y[ig + 3] - opac * (y[ig + 2] - 0.1 * polter);
+  for (int l = 3; l < lg; ++l)
+    f[ig + l] = k / (2.0 * l + 1.0) * (l * y[ig + l - 1] - (l + 1.0) * y[ig + l + 1]) - opac * y[ig + l];
+  f[ig + lg] = k * y[ig + lg - 1] - (lg + 1.0) / tau * y[ig + lg] - opac * y[ig + lg];
+  for (int l = 0; l < lgp; ++l) {
+    T ym = (l > 0) ? y[igp + l - 1] : T(0.0);
+    f[igp + l] = k / (2.0 * l + 1.0) * (l * ym - (l + 1.0) * y[igp + l + 1]) - opac * y[igp + l];
+  }
+  f[igp + 0] = f[igp + 0] + opac * polter / 2.0;
+  f[igp + 2] = f[igp + 2] + opac * polter / 10.0;
+  f[igp + lgp] = k * y[igp + lgp - 1] - (lgp + 1.0) / tau * y[igp + lgp] - opac * y[igp + lgp];
+  const int ir = 9 + lg + lgp;
+  f[ir + 0] = 4.0 / 3.0 * (-thetar - 0.5 * hprime);
+  f[ir + 1] = k2 * (0.25 * deltar - y[ir + 2] / 2.0);
+  f[ir + 2] = 8.0 / 15.0 * (thetar + k2 * alpha) - 0.6 * k * y[ir + 3];
+  for (int l = 3; l < lr; ++l) f[ir + l] = k / (2.0 * l + 1.0) * (l * y[ir + l - 1] - (l + 1.0) * y[ir + l + 1]);
+  f[ir + lr] = k * y[ir + lr - 1] - (lr + 1.0) / tau * y[ir + lr];
+  for (int i = 0; i < nq; ++i) {
+    double qi = m.q[i];
+    T aq = a * c.amnu / qi;
+    T v = 1.0 / sqrt(1.0 + aq * aq);
+    double dlfdlq = -qi / (1.0 + std::exp(-qi));
+    f[iq0 + i] = -k * v * y[iq1 + i] + hprime * dlfdlq / 6.0;
+    f[iq1 + i] = k * v * (y[iq0 + i] - 2.0 * y[iq2 + i]) / 3.0;
+    f[iq2 + i] = k * v * (2.0 * y[iq1 + i] - 3.0 * y[iq3 + i]) / 5.0 - (hprime / 15.0 + 2.0 / 5.0 * etaprime) * dlfdlq;
+    for (int l = 3; l < lnu; ++l)
+      f[iq0 + l * nq + i] = k * v / (2.0 * l + 1.0) *
+                            (l * y[iq0 + (l - 1) * nq + i] - (l + 1.0) * y[iq0 + (l + 1) * nq + i]);
+    f[iq0 + lnu * nq + i] = k * v * y[iq0 + (lnu - 1) * nq + i] - (lnu + 1.0) / tau * y[iq0 + lnu * nq + i];
+  }
+  f[n - 2] = -(1.0 + w_Q) * (thetaq + 0.5 * hprime) - 3.0 * (cs2_Q - w_Q) * aH * deltaq -
+             9.0 * (1.0 + w_Q) * (cs2_Q - ca2_Q) * aH * aH / k2 * thetaq;
+  f[n - 1] = -(1.0 - 3.0 * cs2_Q) * aH * thetaq + cs2_Q / (1.0 + w_Q) * k2 * deltaq;
+}
+
+void outputs(const Model &m, const double *y, double *out) {
+  const deb_cosmo_t &c = m.c;
+  const int lg = m.lmaxg, lgp = m.lmaxgp, lr = m.lmaxr, nq = m.nq;
+  const int n = m.nvar();
+  const double k = m.k, k2 = k * k;
+  const double Omegac = c.Omegam - c.Omegab;
+  const int iq0 = 10 + lg + lgp + lr, iq1 = iq0 + nq;
+  double a = y[0], eta = y[2], deltac = y[3], thetac = y[4], deltab = y[5], thetab = y[6];
+  double deltag = y[7], thetag = y[8];
+  double deltar = y[9 + lg + lgp], thetar = y[10 + lg + lgp];
+  double rhonu = std::exp(spline_eval(m.sp[DEB_SPL_LOGRHONU_LOGA], std::log(a)));
+  double pnu = std::exp(spline_eval(m.sp[DEB_SPL_LOGPNU_LOGA], std::log(a)));
+  double drhonu = 0, fnu = 0;
+  for (int i = 0; i < nq; ++i) {
+    double aq = a * c.amnu / m.q[i];
+    double v = 1.0 / std::sqrt(1.0 + aq * aq);
+    drhonu += m.w[i] * y[iq0 + i] / v;
+    fnu += m.w[i] * y[iq1 + i];
+  }
+  double deltanu = drhonu / rhonu;
+  double thetanu = k * fnu / (rhonu + pnu);
+  double deltaq = y[n - 2], thetaq = y[n - 1];
+  double w_Q = c.w_DE_0 + c.w_DE_a * (1.0 - a);
+  double rho_Q = std::pow(a, -3.0 * (1.0 + c.w_DE_0 + c.w_DE_a)) * std::exp(3.0 * (a - 1.0) * c.w_DE_a);
+  double rppQ = (1.0 + w_Q) * rho_Q * c.grhom * c.OmegaDE * thetaq * a * a;
+  double a2 = a * a;
+  double grho = c.grhom * c.Omegam / a + (c.grhog + c.grhor * (c.Neff + c.Nmnu * rhonu)) / a2 +
+                c.grhom * c.OmegaDE * rho_Q * a2 + c.grhom * c.Omegak;
+  double aH = std::sqrt(grho / 3.0);
+  double dgrho = c.grhom * (Omegac * deltac + c.Omegab * deltab) / a +
+                 (c.grhog * deltag + c.grhor * (c.Neff * deltar + c.Nmnu * drhonu)) / a2 +
+                 c.grhom * c.OmegaDE * deltaq * rho_Q * a2;
+  double dgtheta = c.grhom * (Omegac * thetac + c.Omegab * thetab) / a +
+                   4.0 / 3.0 * (c.grhog * thetag + c.Neff * c.grhor * thetar) / a2 + c.Nmnu * c.grhor * k * fnu / a2 +
+                   rppQ;
+  double hprime = (2.0 * k2 * eta + dgrho) / aH;
+  double etaprime = 0.5 * dgtheta / k2;
+  double alpha = (hprime + 6.0 * etaprime) / 2.0 / k2;
+  double mden = c.grhom * c.Omegam / a + c.grhor * c.Nmnu * rhonu / a2;
+  double deltam = (c.grhom * (Omegac * deltac + c.Omegab * deltab) / a + c.grhor * c.Nmnu * drhonu / a2) / mden;
+  double thetam = (c.grhom * (Omegac * thetac + c.Omegab * thetab) / a + c.Nmnu * c.grhor * k * fnu / a2) / (3.0 * mden);
+  double deltabc = (c.grhom * (Omegac * deltac + c.Omegab * deltab) / a) / (c.grhom * c.Omegam / a);
+  double thetabc = (c.grhom * (Omegac * thetac + c.Omegab * thetab) / a) / (3.0 * (c.grhom * c.Omegam / a) / a2);
+  thetam += alpha * k2;
+  thetabc += alpha * k2;
+  const double o[20] = {eta, etaprime, hprime, alpha, deltam, thetam / aH, deltabc, thetabc / aH,
+                        deltac, thetac / aH, deltab, thetab / aH, deltag, thetag / aH, deltar, thetar / aH,
+                        deltanu, thetanu / aH, deltaq, thetaq / aH};
+  std::memcpy(out, o, sizeof(o));
+}
+
+void ics(const Model &m, double tau, double *y) {
+  const deb_cosmo_t &c = m.c;
+  const int lg = m.lmaxg, lgp = m.lmaxgp, lr = m.lmaxr, nq = m.nq;
+  const int n = m.nvar();
+  const int iq0 = 10 + lg + lgp + lr, iq1 = iq0 + nq, iq2 = iq1 + nq;
+  const double k = m.k;
+  std::fill(y, y + n, 0.0);
+  double a = spline_eval(m.sp[DEB_SPL_A_TAU], tau);
+  double rnu = std::exp(spline_eval(m.sp[DEB_SPL_LOGRHONU_LOGA], std::log(a)));
+  double rhom = c.grhom * c.Omegam / (a * a * a);
+  double rhor = (c.grhog + c.grhor * (c.Neff + c.Nmnu * rnu)) / std::pow(a, 4);
+  double rhonu = c.grhor * (c.Neff + c.Nmnu * rnu) / std::pow(a, 4);
+  double fracb = c.Omegab / c.Omegam;
+  double fracnu = rhonu / rhor;
+  double om = a * rhom / std::sqrt(rhor);
+  const double C = -1.0, s2 = 1.0;
+  double kt = k * tau;
+  double deltag = -(kt * kt) / 3.0 * (1.0 - om * tau / 5.0) * C * s2;
+  double thetag = -(kt * kt * kt) / tau / 36.0 * (1.0 - 3.0 * (1.0 + 5.0 * fracb - fracnu) / 20.0 / (1.0 - fracnu) * om * tau) * C * s2;
+  double deltab = 0.75 * deltag, thetab = thetag, deltac = 0.75 * deltag, thetac = 0.0;
+  double deltar = deltag;
+  double thetar = -(kt * kt * kt * kt) / tau / 36.0 / (4.0 * fracnu + 15.0) *
+                  (4.0 * fracnu + 11.0 + 12.0 - 3.0 * (8.0 * fracnu * fracnu + 50.0 * fracnu + 275.0) / 20.0 / (2.0 * fracnu + 15.0) * tau * om) * C;
+  double shearr = (kt * kt) / (45.0 + 12.0 * fracnu) * (3.0 * s2 - 1.0) * (1.0 + (4.0 * fracnu - 5.0) / 4.0 / (2.0 * fracnu + 15.0) * tau * om) * C;
+  double cs2_Q = c.cs2_DE;
+  double w_Q = c.w_DE_0 + c.w_DE_a * (1.0 - a);
+  double deltaq = (kt * kt) / 4.0 * (1.0 + w_Q) * (4.0 - 3.0 * cs2_Q) / (4.0 - 6.0 * w_Q + 3.0 * cs2_Q) * C * s2;
+  double thetaq = (kt * kt * kt * kt) / tau / 4.0 * cs2_Q / (4.0 - 6.0 * w_Q + 3.0 * cs2_Q) * C * s2;
+  double eta = C * (1.0 - (kt * kt) / 12.0 / (15.0 + 4.0 * fracnu) *
+                              (5.0 + 4.0 * s2 * fracnu - (16.0 * fracnu * fracnu + 280.0 * fracnu + 325.0) / 10.0 / (2.0 * fracnu + 15.0) * tau * om));
+  y[0] = a;
+  y[1] = 0.0;
+  y[2] = eta;
+  y[3] = deltac;
+  y[4] = thetac;
+  y[5] = deltab;
+  y[6] = thetab;
+  y[7] = deltag;
+  y[8] = thetag;
+  y[9 + lg + lgp] = deltar;
+  y[10 + lg + lgp] = thetar;
+  y[11 + lg + lgp] = shearr * 2.0;
+  for (int i = 0; i < nq; ++i) {
+    double qi = m.q[i];
+    double aq = a * c.amnu / qi;
+    double v = 1.0 / std::sqrt(1.0 + aq * aq);
+    double dlfdlq = -qi / (1.0 + std::exp(-qi));
+    y[iq0 + i] = -0.25 * dlfdlq * deltar;
+    y[iq1 + i] = -dlfdlq * thetar / v / k / 3.0;
+    y[iq2 + i] = -0.5 * dlfdlq * shearr;
+  }
+  y[n - 2] = deltaq;
+  y[n - 1] = thetaq;
+}
+
+double start_time(const Model &m, double tau_out_min) {
+  const deb_cosmo_t &c = m.c;
+  double tau0 = c.taumin;
+  double tau1 = spline_eval(m.sp[DEB_SPL_TAU_A], 0.1);
+  double tau_k = 1.0 / m.k;
+  double akthom = 2.3048e-9 * (1.0 - c.YHe) * c.Omegab * c.H0 * c.H0;
+  auto small = [&](double lt) {
+    double tau = std::exp(lt);
+    double xe = spline_eval(m.sp[DEB_SPL_XE_TAU], tau);
+    double a = spline_eval(m.sp[DEB_SPL_A_TAU], tau);
+    double opac = xe * akthom / (a * a);
+    double aH = aprimeoa(m, a);
+    return (1.0 / opac) / (1.0 / aH) / 0.0004 - 1.0;
+  };
+  auto large = [&](double lt) {
+    double tau = std::exp(lt);
+    double a = spline_eval(m.sp[DEB_SPL_A_TAU], tau);
+    double aH = aprimeoa(m, a);
+    return (1.0 / aH) / tau_k / 0.07 - 1.0;
+  };
+  auto bisect = [&](auto fn) {
+    double xl = std::log(tau0), xr = std::log(tau1);
+    for (int i = 0; i < 7; ++i) {
+      double xm = 0.5 * (xl + xr);
+      if (fn(xm) * fn(xl) > 0) xl = xm; else xr = xm;
+    }
+    return 0.5 * (xl + xr);
+  };
+  double lt_large = bisect(large);
+  double lt_small = bisect(small);
+  double ts = std::exp(std::min(lt_small, lt_large));
+  return 0.99 * std::min(tau_out_min, ts);
+}
+
+// ---------------------------------------------------------------------------
+// Rodas5 (W-form) constants, ode_integrators_stiff.py:622-687
+namespace r5 {
+const double gamma = 0.19;
+const double a21 = 2.0, a31 = 3.040894194418781, a32 = 1.041747909077569, a41 = 2.576417536461461,
+             a42 = 1.622083060776640, a43 = -0.9089668560264532, a51 = 2.760842080225597, a52 = 1.446624659844071,
+             a53 = -0.3036980084553738, a54 = 0.2877498600325443, a61 = -14.09640773051259, a62 = 6.925207756232704,
+             a63 = -41.47510893210728, a64 = 2.343771018586405, a65 = 24.13215229196062;
+const double C21 = -10.31323885133993, C31 = -21.04823117650003, C32 = -7.234992135176716, C41 = 32.22751541853323,
+             C42 = -4.943732386540191, C43 = 19.44922031041879, C51 = -20.69865579590063, C52 = -8.816374604402768,
+             C53 = 1.260436877740897, C54 = -0.7495647613787146, C61 = -46.22004352711257, C62 = -17.49534862857472,
+             C63 = -289.6389582892057, C64 = 93.60855400400906, C65 = 318.3822534212147, C71 = 34.20013733472935,
+             C72 = -14.15535402717690, C73 = 57.82335640988400, C74 = 25.83362985412365, C75 = 1.408950972071624,
+             C76 = -6.551835421242162, C81 = 42.57076742291101, C82 = -13.80770672017997, C83 = 93.98938432427124,
+             C84 = 18.77919633714503, C85 = -31.58359187223370, C86 = -6.685968952921985, C87 = -5.810979938412932;
+const double c2 = 0.38, c3 = 0.3878509998321533, c4 = 0.4839718937873840, c5 = 0.4570477008819580;
+const double d1 = gamma, d2 = -0.1823079225333714636, d3 = -0.319231832186874912, d4 = 0.3449828624725343,
+             d5 = -0.377417564392089818;
+}  // namespace r5
+
+struct Work {
+  int n;
+  std::vector<double> J, LU, dT, f, u, rhs, yt;
+  std::vector<int> piv;
+  std::vector<double> K[8];
+  std::vector<Dual> yd, fd;
+  explicit Work(int n_) : n(n_), J(n_ * n_), LU(n_ * n_), dT(n_), f(n_), u(n_), rhs(n_), yt(n_), piv(n_), yd(n_), fd(n_) {
+    for (auto &v : K) v.resize(n_);
+  }
+};
+
+void jacobian(const Model &m, double t, const double *y, Work &w) {
+  const int n = w.n;
+  for (int j = 0; j < n; ++j) {
+    for (int i = 0; i < n; ++i) w.yd[i] = Dual(y[i], i == j ? 1.0 : 0.0);
+    rhs<Dual>(m, Dual(t, 0.0), w.yd.data(), w.fd.data());
+    for (int i = 0; i < n; ++i) w.J[i * n + j] = w.fd[i].d;
+  }
+  for (int i = 0; i < n; ++i) w.yd[i] = Dual(y[i], 0.0);
+  rhs<Dual>(m, Dual(t, 1.0), w.yd.data(), w.fd.data());
+  for (int i = 0; i < n; ++i) w.dT[i] = w.fd[i].d;
+}
+
+// LAPACK getrf-style partial pivoting (first max |.| in the column)
+void lu_factor(double *A, int *piv, int n) {
+  for (int kcol = 0; kcol < n; ++kcol) {
+    int p = kcol;
+    double best = std::fabs(A[kcol * n + kcol]);
+    for (int i = kcol + 1; i < n; ++i) {
+      double v = std::fabs(A[i * n + kcol]);
+      if (v > best) { best = v; p = i; }
+    }
+    piv[kcol] = p;
+    if (p != kcol)
+      for (int j = 0; j < n; ++j) std::swap(A[kcol * n + j], A[p * n + j]);
+    double d = A[kcol * n + kcol];
+    if (d != 0.0) {
+      double inv = 1.0 / d;
+      for (int i = kcol + 1; i < n; ++i) {
+        double l = A[i * n + kcol] * inv;
+        A[i * n + kcol] = l;
+        if (l != 0.0) {
+          const double *rk = A + kcol * n;
+          double *ri = A + i * n;
+          for (int j = kcol + 1; j < n; ++j) ri[j] -= l * rk[j];
+        }
+      }
+    }
+  }
+}
+
+void lu_solve(const double *LU, const int *piv, int n, double *b) {
+  for (int i = 0; i < n; ++i)
+    if (piv[i] != i) std::swap(b[i], b[piv[i]]);
+  for (int i = 1; i < n; ++i) {
+    double s = b[i];
+    const double *r = LU + i * n;
+    for (int j = 0; j < i; ++j) s -= r[j] * b[j];
+    b[i] = s;
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    double s = b[i];
+    const double *r = LU + i * n;
+    for (int j = i + 1; j < n; ++j) s -= r[j] * b[j];
+    b[i] = s / r[i];
+  }
+}
+
+// one Rodas5Transformed step; returns y1 in y1, error estimate (k8) in err
+void rodas5_step(const Model &m, double t0, double t1, const double *y0, double *y1, double *err, Work &w) {
+  using namespace r5;
+  const int n = w.n;
+  const double dt = t1 - t0;
+  jacobian(m, t0, y0, w);
+  const double dtg = dt * gamma;
+  for (int i = 0; i < n * n; ++i) w.LU[i] = -w.J[i];
+  for (int i = 0; i < n; ++i) w.LU[i * n + i] += 1.0 / dtg;
+  lu_factor(w.LU.data(), w.piv.data(), n);
+  auto &k1 = w.K[0], &k2 = w.K[1], &k3 = w.K[2], &k4 = w.K[3], &k5 = w.K[4], &k6 = w.K[5], &k7 = w.K[6], &k8 = w.K[7];
+  auto F = [&](double t, const double *u) { rhs<double>(m, t, u, w.f.data()); };
+  auto solve = [&](std::vector<double> &k) {
+    std::copy(w.rhs.begin(), w.rhs.end(), k.begin());
+    lu_solve(w.LU.data(), w.piv.data(), n, k.data());
+  };
+  F(t0, y0);
+  for (int i = 0; i < n; ++i) w.rhs[i] = w.f[i] + dt * d1 * w.dT[i];
+  solve(k1);
+  for (int i = 0; i < n; ++i) w.u[i] = y0[i] + a21 * k1[i];
+  F(t0 + c2 * dt, w.u.data());
+  for (int i = 0; i < n; ++i) w.rhs[i] = w.f[i] + dt * d2 * w.dT[i] + (C21 / dt) * k1[i];
+  solve(k2);
+  for (int i = 0; i < n; ++i) w.u[i] = y0[i] + a31 * k1[i] + a32 * k2[i];
+  F(t0 + c3 * dt, w.u.data());
+  for (int i = 0; i < n; ++i) w.rhs[i] = w.f[i] + dt * d3 * w.dT[i] + ((C31 / dt) * k1[i] + (C32 / dt) * k2[i]);
+  solve(k3);
+  for (int i = 0; i < n; ++i) w.u[i] = y0[i] + a41 * k1[i] + a42 * k2[i] + a43 * k3[i];
+  F(t0 + c4 * dt, w.u.data());
+  for (int i = 0; i < n; ++i)
+    w.rhs[i] = w.f[i] + dt * d4 * w.dT[i] + ((C41 / dt) * k1[i] + (C42 / dt) * k2[i] + (C43 / dt) * k3[i]);
+  solve(k4);
+  for (int i = 0; i < n; ++i) w.u[i] = y0[i] + a51 * k1[i] + a52 * k2[i] + a53 * k3[i] + a54 * k4[i];
+  F(t0 + c5 * dt, w.u.data());
+  for (int i = 0; i < n; ++i)
+    w.rhs[i] = w.f[i] + dt * d5 * w.dT[i] +
+               ((C51 / dt) * k1[i] + (C52 / dt) * k2[i] + (C53 / dt) * k3[i] + (C54 / dt) * k4[i]);
+  solve(k5);
+  for (int i = 0; i < n; ++i) w.u[i] = y0[i] + a61 * k1[i] + a62 * k2[i] + a63 * k3[i] + a64 * k4[i] + a65 * k5[i];
+  F(t0 + dt, w.u.data());
+  for (int i = 0; i < n; ++i)
+    w.rhs[i] = w.f[i] + ((C61 / dt) * k1[i] + (C62 / dt) * k2[i] + (C63 / dt) * k3[i] + (C64 / dt) * k4[i] +
+                         (C65 / dt) * k5[i]);
+  solve(k6);
+  for (int i = 0; i < n; ++i) w.u[i] = w.u[i] + k6[i];
+  F(t0 + dt, w.u.data());
+  for (int i = 0; i < n; ++i)
+    w.rhs[i] = w.f[i] + ((C71 / dt) * k1[i] + (C72 / dt) * k2[i] + (C73 / dt) * k3[i] + (C74 / dt) * k4[i] +
+                         (C75 / dt) * k5[i] + (C76 / dt) * k6[i]);
+  solve(k7);
+  for (int i = 0; i < n; ++i) w.u[i] = w.u[i] + k7[i];
+  F(t0 + dt, w.u.data());
+  for (int i = 0; i < n; ++i)
+    w.rhs[i] = w.f[i] + ((C81 / dt) * k1[i] + (C82 / dt) * k2[i] + (C83 / dt) * k3[i] + (C84 / dt) * k4[i] +
+                         (C85 / dt) * k5[i] + (C86 / dt) * k6[i] + (C87 / dt) * k7[i]);
+  solve(k8);
+  for (int i = 0; i < n; ++i) {
+    y1[i] = w.u[i] + k8[i];
+    err[i] = k8[i];
+  }
+}
+
+struct SolverCfg {
+  double rtol, atol, pcoeff, icoeff, dcoeff, factormax, factormin;
+  int max_steps;
+  int return_full;
+};
+
+// evolve_one_mode: returns status; fills out[nout * (return_full ? n : 20)]
+int evolve_mode(const Model &m, const SolverCfg &s, const double *tau_out, int nout, double *out, int *nsteps,
+                int *naccept, double *tau_start_out) {
+  const int n = m.nvar();
+  Work w(n);
+  double tmin = tau_out[0];
+  for (int i = 1; i < nout; ++i) tmin = std::min(tmin, tau_out[i]);
+  double tmax = tau_out[0];
+  for (int i = 1; i < nout; ++i) tmax = std::max(tmax, tau_out[i]);
+  const double t0 = start_time(m, tmin);
+  if (tau_start_out) *tau_start_out = t0;
+  const double t1 = tmax;
+  std::vector<double> y(n), y1(n), err(n), ys(n);
+  ics(m, t0, y.data());
+  const double dt0 = std::min(t0 / 4.0, 0.5 * (t1 - t0));
+  double tprev = t0, tnext = std::min(t0 + dt0, t1);
+  double prev_inv = 1.0, prev_prev_inv = 1.0;
+  const double eo = 5.0;
+  const double coeff1 = (s.icoeff + s.pcoeff + s.dcoeff) / eo;
+  const double coeff2 = -(s.pcoeff + 2.0 * s.dcoeff) / eo;
+  const double coeff3 = s.dcoeff / eo;
+  const int fidx[6] = {0, 2, 3, 5, 6, 7};
+  const double fw[6] = {1.0, m.k * m.k, 1.0, 1.0, 1.0 / (m.k * m.k), 1.0};
+  int iout = 0, steps = 0, acc = 0;
+  const int width = s.return_full ? n : 20;
+  int status = DEB_STATUS_OK;
+  while (tprev < t1) {
+    if (steps >= s.max_steps) { status = DEB_STATUS_MAX_STEPS; break; }
+    rodas5_step(m, tprev, tnext, y.data(), y1.data(), err.data(), w);
+    double ss = 0.0;
+    for (int j = 0; j < 6; ++j) {
+      int i = fidx[j];
+      double sc = err[i] / (s.atol + std::max(std::fabs(y[i]), std::fabs(y1[i])) * s.rtol);
+      double x = sc * fw[j];
+      ss += x * x;
+    }
+    double nrm = std::sqrt(ss / 6.0);
+    if (!std::isfinite(nrm)) { status = DEB_STATUS_NONFINITE; break; }
+    bool keep = nrm < 1.0;
+    double inv = 1.0 / nrm;
+    double f1 = coeff1 == 0.0 ? 1.0 : std::pow(inv, coeff1);
+    double f2 = coeff2 == 0.0 ? 1.0 : std::pow(prev_inv, coeff2);
+    double f3 = coeff3 == 0.0 ? 1.0 : std::pow(prev_prev_inv, coeff3);
+    double fmin = keep ? 1.0 : s.factormin;
+    double factor = std::min(std::max(0.9 * f1 * f2 * f3, fmin), s.factormax);
+    double dt = (tnext - tprev) * factor;
+    if (inv == 0.0 || std::isinf(inv)) inv = 1.0;
+    if (keep) {
+      while (iout < nout && tau_out[iout] <= tnext) {
+        double th = (tau_out[iout] - tprev) / (tnext - tprev);
+        for (int i = 0; i < n; ++i) ys[i] = y[i] + th * (y1[i] - y[i]);
+        if (s.return_full) std::copy(ys.begin(), ys.end(), out + (size_t)iout * width);
+        else outputs(m, ys.data(), out + (size_t)iout * width);
+        ++iout;
+      }
+      prev_prev_inv = prev_inv;
+      prev_inv = inv;
+      y.swap(y1);
+      ++acc;
+    }
+    double nt0 = keep ? tnext : tprev;
+    double nt1 = nt0 + dt;
+    if (nt1 > t1 - 1e-10) nt1 = keep ? t1 : nt0 + 0.5 * (t1 - nt0);
+    tprev = nt0;
+    tnext = nt1;
+    ++steps;
+  }
+  *nsteps = steps;
+  *naccept = acc;
+  return status;
+}
+
+Model make_model(const deb_cosmo_t *c, const double *tables, const int32_t *desc, const deb_config_t *cfg,
+                 const double *q, const double *w, double k) {
+  Model m;
+  m.c = *c;
+  for (int s = 0; s < DEB_NSPLINE; ++s) {
+    int off = desc[2 * s], n = desc[2 * s + 1];
+    m.sp[s] = {tables + off, tables + off + n, tables + off + 2 * n, n};
+  }
+  m.k = k;
+  m.lmaxg = cfg->lmaxg;
+  m.lmaxgp = cfg->lmaxgp;
+  m.lmaxr = cfg->lmaxr;
+  m.lmaxnu = cfg->lmaxnu;
+  m.nq = cfg->nqmax;
+  for (int i = 0; i < m.nq; ++i) {
+    m.q[i] = q[i];
+    m.w[i] = w[i];
+  }
+  return m;
+}
+
+}  // namespace
+
+extern "C" {
+
+int oracle_nvar(const deb_config_t *cfg) {
+  return 12 + cfg->lmaxg + cfg->lmaxgp + cfg->lmaxr + cfg->nqmax * (cfg->lmaxnu + 1);
+}
+
+// f = model_synchronous(tau, y) for one mode
+int oracle_rhs(const deb_config_t *cfg, const deb_cosmo_t *c, const double *tables, const int32_t *desc,
+               const double *q, const double *w, double k, double tau, const double *y, double *f) {
+  Model m = make_model(c, tables, desc, cfg, q, w, k);
+  rhs<double>(m, tau, y, f);
+  return 0;
+}
+
+// exact dense Jacobian J[n*n] (row-major) and dT = df/dtau
+int oracle_jacobian(const deb_config_t *cfg, const deb_cosmo_t *c, const double *tables, const int32_t *desc,
+                    const double *q, const double *w, double k, double tau, const double *y, double *J, double *dT) {
+  Model m = make_model(c, tables, desc, cfg, q, w, k);
+  Work wk(m.nvar());
+  jacobian(m, tau, y, wk);
+  std::copy(wk.J.begin(), wk.J.end(), J);
+  std::copy(wk.dT.begin(), wk.dT.end(), dT);
+  return 0;
+}
+
+int oracle_rodas5_step(const deb_config_t *cfg, const deb_cosmo_t *c, const double *tables, const int32_t *desc,
+                       const double *q, const double *w, double k, double t0, double t1, const double *y0, double *y1,
+                       double *err) {
+  Model m = make_model(c, tables, desc, cfg, q, w, k);
+  Work wk(m.nvar());
+  rodas5_step(m, t0, t1, y0, y1, err, wk);
+  return 0;
+}
+
+int oracle_ics(const deb_config_t *cfg, const deb_cosmo_t *c, const double *tables, const int32_t *desc,
+               const double *q, const double *w, double k, double tau_out_min, double *tau_start, double *y0) {
+  Model m = make_model(c, tables, desc, cfg, q, w, k);
+  *tau_start = start_time(m, tau_out_min);
+  ics(m, *tau_start, y0);
+  return 0;
+}
+
+int oracle_outputs(const deb_config_t *cfg, const deb_cosmo_t *c, const double *tables, const int32_t *desc,
+                   const double *q, const double *w, double k, const double *y, double *out20) {
+  Model m = make_model(c, tables, desc, cfg, q, w, k);
+  outputs(m, y, out20);
+  return 0;
+}
+
+// Full per-mode solve for nk modes of one cosmology, threaded over modes.
+int oracle_evolve(const deb_config_t *cfg, const deb_cosmo_t *c, const double *tables, const int32_t *desc,
+                  const double *q, const double *w, int nk, const double *kmodes, int nout, const double *tau_out,
+                  double *y_out, int32_t *status, int32_t *nsteps, int32_t *naccept, double *tau_start,
+                  int nthreads) {
+  SolverCfg s{cfg->rtol, cfg->atol, cfg->pcoeff, cfg->icoeff, cfg->dcoeff, cfg->factormax, cfg->factormin,
+              cfg->max_steps, cfg->return_full};
+  const int n = oracle_nvar(cfg);
+  const size_t width = cfg->return_full ? (size_t)n : 20;
+  std::atomic<int> next(0);
+  auto worker = [&]() {
+    for (;;) {
+      int i = next.fetch_add(1);
+      if (i >= nk) break;
+      Model m = make_model(c, tables, desc, cfg, q, w, kmodes[i]);
+      int st, na;
+      double ts;
+      status[i] = evolve_mode(m, s, tau_out, nout, y_out + (size_t)i * nout * width, &st, &na, &ts);
+      nsteps[i] = st;
+      naccept[i] = na;
+      if (tau_start) tau_start[i] = ts;
+    }
+  };
+  if (nthreads <= 1) {
+    worker();
+  } else {
+    std::vector<std::thread> th;
+    for (int t = 0; t < nthreads; ++t) th.emplace_back(worker);
+    for (auto &t : th) t.join();
+  }
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,114 @@
+"""ctypes bindings of the CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+Only tests/, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` may import this module.  It wraps ``liboracle.so`` built from
+``deb_oracle.cpp`` (a plain C++ restatement of the reference's
+``evolve_perturbations`` path; see that file's header for the reference
+file:line map).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def build(force: bool = False) -> str:
+    path = os.path.join(_HERE, "liboracle.so")
+    if force or not os.path.exists(path):
+        subprocess.run(["make", "-C", _HERE, "-B" if force else "all"], check=True, capture_output=True)
+    return path
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        _LIB = ctypes.CDLL(path)
+    return _LIB
+
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int32)
+
+
+def _p(a, t=_dp):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+class Problem:
+    """One cosmology's packed inputs for the oracle (reals only)."""
+
+    def __init__(self, param, cfg, b: int = 0):
+        import sys
+        pkg = os.path.join(os.path.dirname(_HERE), "disco-eb_amd")
+        if pkg not in sys.path:
+            sys.path.insert(0, pkg)
+        from discoeb_amd import packing
+        self.cfg = cfg
+        self.cosmo = np.ascontiguousarray(packing.pack_cosmo(param)[b])
+        tab, desc = packing.pack_tables(param)
+        self.tables = np.ascontiguousarray(tab[b])
+        self.desc = np.ascontiguousarray(desc)
+        self.q, self.w = packing.nu_nodes(cfg.nqmax)
+        if self.q.size == 0:
+            self.q = np.zeros(1)
+            self.w = np.zeros(1)
+        self.n = packing.nvar(cfg.lmaxg, cfg.lmaxgp, cfg.lmaxr, cfg.lmaxnu, cfg.nqmax)
+
+    def _common(self):
+        return (ctypes.byref(self.cfg), _p(self.cosmo), _p(self.tables), _p(self.desc, _ip), _p(self.q), _p(self.w))
+
+    def rhs(self, k, tau, y):
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        f = np.zeros(self.n)
+        lib().oracle_rhs(*self._common(), ctypes.c_double(k), ctypes.c_double(tau), _p(y), _p(f))
+        return f
+
+    def jacobian(self, k, tau, y):
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        J = np.zeros((self.n, self.n))
+        dT = np.zeros(self.n)
+        lib().oracle_jacobian(*self._common(), ctypes.c_double(k), ctypes.c_double(tau), _p(y), _p(J), _p(dT))
+        return J, dT
+
+    def rodas5_step(self, k, t0, t1, y0):
+        y0 = np.ascontiguousarray(y0, dtype=np.float64)
+        y1 = np.zeros(self.n)
+        err = np.zeros(self.n)
+        lib().oracle_rodas5_step(*self._common(), ctypes.c_double(k), ctypes.c_double(t0), ctypes.c_double(t1),
+                                 _p(y0), _p(y1), _p(err))
+        return y1, err
+
+    def ics(self, k, tau_out_min):
+        ts = ctypes.c_double(0.0)
+        y0 = np.zeros(self.n)
+        lib().oracle_ics(*self._common(), ctypes.c_double(k), ctypes.c_double(tau_out_min), ctypes.byref(ts), _p(y0))
+        return ts.value, y0
+
+    def outputs(self, k, y):
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        o = np.zeros(20)
+        lib().oracle_outputs(*self._common(), ctypes.c_double(k), _p(y), _p(o))
+        return o
+
+    def evolve(self, kmodes, tau_out, nthreads: int = 1):
+        kmodes = np.ascontiguousarray(kmodes, dtype=np.float64)
+        tau_out = np.ascontiguousarray(np.atleast_1d(tau_out), dtype=np.float64)
+        nk, nout = kmodes.size, tau_out.size
+        width = self.n if self.cfg.return_full else 20
+        y = np.zeros((nk, nout, width))
+        st = np.zeros(nk, dtype=np.int32)
+        ns = np.zeros(nk, dtype=np.int32)
+        na = np.zeros(nk, dtype=np.int32)
+        ts = np.zeros(nk)
+        lib().oracle_evolve(*self._common(), ctypes.c_int(nk), _p(kmodes), ctypes.c_int(nout), _p(tau_out), _p(y),
+                            _p(st, _ip), _p(ns, _ip), _p(na, _ip), _p(ts), ctypes.c_int(nthreads))
+        return dict(y=y, status=st, nsteps=ns, naccept=na, tau_start=ts)
