@@ -224,6 +224,12 @@ void rhs(const Model &m, T tau, const T *y, T *f) {
   f[n - 2] = -(1.0 + w_Q) * (thetaq + 0.5 * hprime) - 3.0 * (cs2_Q - w_Q) * aH * deltaq -
              9.0 * (1.0 + w_Q) * (cs2_Q - ca2_Q) * aH * aH / k2 * thetaq;
   f[n - 1] = -(1.0 - 3.0 * cs2_Q) * aH * thetaq + cs2_Q / (1.0 + w_Q) * k2 * deltaq;
+  if (c.w_DE_0 == -1.0 && c.w_DE_a == 0.0) {
+    // cosmological constant: the fluid equations are singular at w = -1
+    // (inf * 0 in theta_Q'); their w -> -1 limit keeps delta_Q = theta_Q = 0
+    f[n - 2] = T(0.0);
+    f[n - 1] = T(0.0);
+  }
 }
 
 void outputs(const Model &m, const double *y, double *out) {
@@ -330,6 +336,10 @@ void ics(const Model &m, double tau, double *y) {
   }
   y[n - 2] = deltaq;
   y[n - 1] = thetaq;
+  if (c.w_DE_0 == -1.0 && c.w_DE_a == 0.0) {
+    y[n - 2] = 0.0;
+    y[n - 1] = 0.0;
+  }
 }
 
 double start_time(const Model &m, double tau_out_min) {
@@ -454,18 +464,43 @@ void lu_solve(const double *LU, const int *piv, int n, double *b) {
   }
 }
 
+// problem adaptor: the perturbation system of one mode
+struct ModeSystem {
+  const Model &m;
+  void f(double t, const double *y, double *out) const { rhs<double>(m, t, y, out); }
+  void jac(double t, const double *y, Work &w) const { jacobian(m, t, y, w); }
+};
+
+// linear test system f = A y + g0 + g1 t (pins the step on closed forms)
+struct LinearSystem {
+  int n;
+  const double *A, *g0, *g1;
+  void f(double t, const double *y, double *out) const {
+    for (int i = 0; i < n; ++i) {
+      double s = g0[i] + g1[i] * t;
+      for (int j = 0; j < n; ++j) s += A[i * n + j] * y[j];
+      out[i] = s;
+    }
+  }
+  void jac(double, const double *, Work &w) const {
+    for (int i = 0; i < n * n; ++i) w.J[i] = A[i];
+    for (int i = 0; i < n; ++i) w.dT[i] = g1[i];
+  }
+};
+
 // one Rodas5Transformed step; returns y1 in y1, error estimate (k8) in err
-void rodas5_step(const Model &m, double t0, double t1, const double *y0, double *y1, double *err, Work &w) {
+template <typename Sys>
+void rodas5_step_sys(const Sys &sys, double t0, double t1, const double *y0, double *y1, double *err, Work &w) {
   using namespace r5;
   const int n = w.n;
   const double dt = t1 - t0;
-  jacobian(m, t0, y0, w);
+  sys.jac(t0, y0, w);
   const double dtg = dt * gamma;
   for (int i = 0; i < n * n; ++i) w.LU[i] = -w.J[i];
   for (int i = 0; i < n; ++i) w.LU[i * n + i] += 1.0 / dtg;
   lu_factor(w.LU.data(), w.piv.data(), n);
   auto &k1 = w.K[0], &k2 = w.K[1], &k3 = w.K[2], &k4 = w.K[3], &k5 = w.K[4], &k6 = w.K[5], &k7 = w.K[6], &k8 = w.K[7];
-  auto F = [&](double t, const double *u) { rhs<double>(m, t, u, w.f.data()); };
+  auto F = [&](double t, const double *u) { sys.f(t, u, w.f.data()); };
   auto solve = [&](std::vector<double> &k) {
     std::copy(w.rhs.begin(), w.rhs.end(), k.begin());
     lu_solve(w.LU.data(), w.piv.data(), n, k.data());
@@ -514,6 +549,10 @@ void rodas5_step(const Model &m, double t0, double t1, const double *y0, double 
     y1[i] = w.u[i] + k8[i];
     err[i] = k8[i];
   }
+}
+
+void rodas5_step(const Model &m, double t0, double t1, const double *y0, double *y1, double *err, Work &w) {
+  rodas5_step_sys(ModeSystem{m}, t0, t1, y0, y1, err, w);
 }
 
 struct SolverCfg {
@@ -619,6 +658,33 @@ Model make_model(const deb_cosmo_t *c, const double *tables, const int32_t *desc
 
 extern "C" {
 
+// Rodas5 W-form tableau: a[8][8] (row i: stage-i combination of k_j; row 8
+// also gives y1 = u8 + k8), C[8][8], c[8], d[8], gamma
+int oracle_rodas5_tableau(double *a, double *C, double *c, double *d, double *gamma) {
+  using namespace r5;
+  std::fill(a, a + 64, 0.0);
+  std::fill(C, C + 64, 0.0);
+  const double A[8][8] = {{0}, {a21}, {a31, a32}, {a41, a42, a43}, {a51, a52, a53, a54},
+                          {a61, a62, a63, a64, a65}, {a61, a62, a63, a64, a65, 1.0},
+                          {a61, a62, a63, a64, a65, 1.0, 1.0}};
+  const double CC[8][8] = {{0}, {C21}, {C31, C32}, {C41, C42, C43}, {C51, C52, C53, C54},
+                           {C61, C62, C63, C64, C65}, {C71, C72, C73, C74, C75, C76},
+                           {C81, C82, C83, C84, C85, C86, C87}};
+  for (int i = 0; i < 8; ++i)
+    for (int j = 0; j < 8; ++j) {
+      a[i * 8 + j] = A[i][j];
+      C[i * 8 + j] = CC[i][j];
+    }
+  const double cc[8] = {0.0, c2, c3, c4, c5, 1.0, 1.0, 1.0};
+  const double dd[8] = {d1, d2, d3, d4, d5, 0.0, 0.0, 0.0};
+  for (int i = 0; i < 8; ++i) {
+    c[i] = cc[i];
+    d[i] = dd[i];
+  }
+  *gamma = r5::gamma;
+  return 0;
+}
+
 int oracle_nvar(const deb_config_t *cfg) {
   return 12 + cfg->lmaxg + cfg->lmaxgp + cfg->lmaxr + cfg->nqmax * (cfg->lmaxnu + 1);
 }
@@ -656,6 +722,22 @@ int oracle_ics(const deb_config_t *cfg, const deb_cosmo_t *c, const double *tabl
   Model m = make_model(c, tables, desc, cfg, q, w, k);
   *tau_start = start_time(m, tau_out_min);
   ics(m, *tau_start, y0);
+  return 0;
+}
+
+// one Rodas5 step of the linear system y' = A y + g0 + g1 t (same step code)
+int oracle_rodas5_linear(int n, const double *A, const double *g0, const double *g1, double t0, double t1,
+                         const double *y0, double *y1, double *err) {
+  Work wk(n);
+  rodas5_step_sys(LinearSystem{n, A, g0, g1}, t0, t1, y0, y1, err, wk);
+  return 0;
+}
+
+// adiabatic initial conditions at an explicit time tau
+int oracle_ics_at(const deb_config_t *cfg, const deb_cosmo_t *c, const double *tables, const int32_t *desc,
+                  const double *q, const double *w, double k, double tau, double *y0) {
+  Model m = make_model(c, tables, desc, cfg, q, w, k);
+  ics(m, tau, y0);
   return 0;
 }
 

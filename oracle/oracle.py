@@ -93,6 +93,11 @@ class Problem:
         lib().oracle_ics(*self._common(), ctypes.c_double(k), ctypes.c_double(tau_out_min), ctypes.byref(ts), _p(y0))
         return ts.value, y0
 
+    def ics_at(self, k, tau):
+        y0 = np.zeros(self.n)
+        lib().oracle_ics_at(*self._common(), ctypes.c_double(k), ctypes.c_double(tau), _p(y0))
+        return y0
+
     def outputs(self, k, y):
         y = np.ascontiguousarray(y, dtype=np.float64)
         o = np.zeros(20)
