@@ -22,74 +22,116 @@
 //
 // The Jacobian is exact (forward-mode duals, like jax.jacfwd) and the linear
 // algebra is a dense partial-pivot LU (like jax.scipy.linalg.lu_factor).
+//
+// Gradients (the reference differentiates the whole pipeline with jax.jacfwd,
+// nb_minimal_example.ipynb:297): every function is templated on the scalar
+// type, so the same code runs on nested dual numbers.  The tangent solve
+// (oracle_evolve_tangent) replays the accepted steps of the plain solve with
+// the step-size controller's decisions held fixed: step n starts at
+// t_n = t0 + s_n (t1 - t0) with the fractions s_n of the plain solve, while
+// t0 (start time), t1 (last output time), the initial state, every cosmology
+// scalar and every spline knot carry the tangent.  The result is the exact
+// directional derivative of the discrete solution.
 #include <algorithm>
 #include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 #include "../include/discoeb_hip.h"
 
 namespace {
 
-struct Dual {
-  double v, d;
-  Dual(double v_ = 0.0, double d_ = 0.0) : v(v_), d(d_) {}
+// ---------------------------------------------------------------------------
+// nestable forward-mode dual numbers: D<double> carries one tangent,
+// D<D<double>> a derivative whose value and slope both carry a tangent
+template <class B>
+struct D {
+  B v, d;
+  D() : v(0.0), d(0.0) {}
+  D(const B &v_) : v(v_), d(0.0) {}
+  D(const B &v_, const B &d_) : v(v_), d(d_) {}
+  template <class X, class = std::enable_if_t<std::is_arithmetic_v<X> && !std::is_same_v<X, B>>>
+  D(X x) : v(B(double(x))), d(0.0) {}
 };
-inline Dual operator+(Dual a, Dual b) { return {a.v + b.v, a.d + b.d}; }
-inline Dual operator-(Dual a, Dual b) { return {a.v - b.v, a.d - b.d}; }
-inline Dual operator-(Dual a) { return {-a.v, -a.d}; }
-inline Dual operator*(Dual a, Dual b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
-inline Dual operator/(Dual a, Dual b) {
-  double q = a.v / b.v;
+
+template <class S, class B>
+constexpr bool kScalarOf = std::is_same_v<S, double> || std::is_same_v<S, int> || std::is_same_v<S, B>;
+
+template <class B>
+D<B> operator+(const D<B> &a, const D<B> &b) { return {a.v + b.v, a.d + b.d}; }
+template <class B>
+D<B> operator-(const D<B> &a, const D<B> &b) { return {a.v - b.v, a.d - b.d}; }
+template <class B>
+D<B> operator-(const D<B> &a) { return {-a.v, -a.d}; }
+template <class B>
+D<B> operator*(const D<B> &a, const D<B> &b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+template <class B>
+D<B> operator/(const D<B> &a, const D<B> &b) {
+  B q = a.v / b.v;
   return {q, (a.d - q * b.d) / b.v};
 }
-inline Dual operator+(Dual a, double b) { return {a.v + b, a.d}; }
-inline Dual operator+(double b, Dual a) { return {a.v + b, a.d}; }
-inline Dual operator-(Dual a, double b) { return {a.v - b, a.d}; }
-inline Dual operator-(double b, Dual a) { return {b - a.v, -a.d}; }
-inline Dual operator*(Dual a, double b) { return {a.v * b, a.d * b}; }
-inline Dual operator*(double b, Dual a) { return {a.v * b, a.d * b}; }
-inline Dual operator/(Dual a, double b) { return {a.v / b, a.d / b}; }
-inline Dual operator/(double b, Dual a) {
-  double q = b / a.v;
+template <class B, class S, class = std::enable_if_t<kScalarOf<S, B>>>
+D<B> operator+(const D<B> &a, const S &b) { return {a.v + b, a.d}; }
+template <class B, class S, class = std::enable_if_t<kScalarOf<S, B>>>
+D<B> operator+(const S &b, const D<B> &a) { return {b + a.v, a.d}; }
+template <class B, class S, class = std::enable_if_t<kScalarOf<S, B>>>
+D<B> operator-(const D<B> &a, const S &b) { return {a.v - b, a.d}; }
+template <class B, class S, class = std::enable_if_t<kScalarOf<S, B>>>
+D<B> operator-(const S &b, const D<B> &a) { return {b - a.v, -a.d}; }
+template <class B, class S, class = std::enable_if_t<kScalarOf<S, B>>>
+D<B> operator*(const D<B> &a, const S &b) { return {a.v * b, a.d * b}; }
+template <class B, class S, class = std::enable_if_t<kScalarOf<S, B>>>
+D<B> operator*(const S &b, const D<B> &a) { return {b * a.v, b * a.d}; }
+template <class B, class S, class = std::enable_if_t<kScalarOf<S, B>>>
+D<B> operator/(const D<B> &a, const S &b) { return {a.v / b, a.d / b}; }
+template <class B, class S, class = std::enable_if_t<kScalarOf<S, B>>>
+D<B> operator/(const S &b, const D<B> &a) {
+  B q = b / a.v;
   return {q, -q * a.d / a.v};
 }
-inline Dual exp(Dual a) {
-  double e = std::exp(a.v);
-  return {e, e * a.d};
-}
-inline Dual log(Dual a) { return {std::log(a.v), a.d / a.v}; }
-inline Dual sqrt(Dual a) {
-  double s = std::sqrt(a.v);
-  return {s, 0.5 * a.d / s};
-}
-inline Dual pow(Dual a, double p) {
-  double v = std::pow(a.v, p);
-  return {v, p * std::pow(a.v, p - 1.0) * a.d};
-}
-inline double val(double x) { return x; }
-inline double val(Dual x) { return x.v; }
 using std::exp;
 using std::log;
 using std::pow;
 using std::sqrt;
+template <class B>
+D<B> exp(const D<B> &a) {
+  B e = exp(a.v);
+  return {e, e * a.d};
+}
+template <class B>
+D<B> log(const D<B> &a) { return {log(a.v), a.d / a.v}; }
+template <class B>
+D<B> sqrt(const D<B> &a) {
+  B s = sqrt(a.v);
+  return {s, 0.5 * a.d / s};
+}
+template <class B>
+D<B> pow(const D<B> &a, double p) { return {pow(a.v, p), p * pow(a.v, p - 1.0) * a.d}; }
+inline double val(double x) { return x; }
+template <class B>
+double val(const D<B> &x) { return val(x.v); }
+
+using Dd = D<double>;
 
 // ---------------------------------------------------------------------------
 // natural cubic spline evaluation: interval = clip(searchsorted_left(x, v)-1, 0, n-2)
-struct SplineView {
-  const double *x, *y, *S;
+template <class P>
+struct SplineT {
+  const P *x, *y, *S;
   int n;
 };
 
-inline int interval(const SplineView &s, double v) {
+template <class P>
+int interval(const SplineT<P> &s, double v) {
   // number of knots strictly below v (left-sided searchsorted)
   int lo = 0, hi = s.n;
   while (lo < hi) {
     int mid = (lo + hi) >> 1;
-    if (s.x[mid] < v) lo = mid + 1; else hi = mid;
+    if (val(s.x[mid]) < v) lo = mid + 1; else hi = mid;
   }
   int i = lo - 1;
   if (i < 0) i = 0;
@@ -97,65 +139,94 @@ inline int interval(const SplineView &s, double v) {
   return i;
 }
 
-template <typename T>
-T spline_eval(const SplineView &s, T xv) {
+template <class T, class P>
+T spline_eval(const SplineT<P> &s, const T &xv) {
   int i = interval(s, val(xv));
-  double h = s.x[i + 1] - s.x[i];
+  P h = s.x[i + 1] - s.x[i];
   T t = (xv - s.x[i]) / h;
   T A = 1.0 - t;
   T B = t;
-  return A * s.y[i] + B * s.y[i + 1] + ((A * A * A - A) * s.S[i] + (B * B * B - B) * s.S[i + 1]) * (h * h) / 6.0;
+  return A * s.y[i] + B * s.y[i + 1] + ((A * A * A - A) * s.S[i] + (B * B * B - B) * s.S[i + 1]) * ((h * h) / 6.0);
 }
 
-struct Model {
-  deb_cosmo_t c;
-  SplineView sp[DEB_NSPLINE];
+template <class P>
+struct CosmoT {
+  P grhom, grhog, grhor, Omegam, Omegab, OmegaDE, Omegak, Neff, Nmnu, amnu, w_DE_0, w_DE_a, cs2_DE, YHe, H0,
+      taumin;
+};
+
+CosmoT<double> cosmo_of(const deb_cosmo_t &c) {
+  return {c.grhom, c.grhog, c.grhor, c.Omegam, c.Omegab, c.OmegaDE, c.Omegak, c.Neff, c.Nmnu, c.amnu,
+          c.w_DE_0, c.w_DE_a, c.cs2_DE, c.YHe, c.H0, c.taumin};
+}
+
+CosmoT<Dd> cosmo_of(const deb_cosmo_t &c, const deb_cosmo_t &t) {
+  return {{c.grhom, t.grhom}, {c.grhog, t.grhog}, {c.grhor, t.grhor}, {c.Omegam, t.Omegam},
+          {c.Omegab, t.Omegab}, {c.OmegaDE, t.OmegaDE}, {c.Omegak, t.Omegak}, {c.Neff, t.Neff},
+          {c.Nmnu, t.Nmnu}, {c.amnu, t.amnu}, {c.w_DE_0, t.w_DE_0}, {c.w_DE_a, t.w_DE_a},
+          {c.cs2_DE, t.cs2_DE}, {c.YHe, t.YHe}, {c.H0, t.H0}, {c.taumin, t.taumin}};
+}
+
+template <class P>
+struct ModelT {
+  CosmoT<P> c;
+  SplineT<P> sp[DEB_NSPLINE];
   double k;
   int lmaxg, lmaxgp, lmaxr, lmaxnu, nq;
   double q[DEB_MAXNQ], w[DEB_MAXNQ];
+  bool lambda;  // w0 == -1 and wa == 0
   int nvar() const { return 12 + lmaxg + lmaxgp + lmaxr + nq * (lmaxnu + 1); }
 };
+using Model = ModelT<double>;
 
-template <typename T>
-T aprimeoa(const Model &m, T a) {
-  const deb_cosmo_t &c = m.c;
+// rho_Q(a) = a^{-3(1+w0+wa)} exp(3 (a-1) wa), with the exponent's tangent
+template <class T, class P>
+T rho_de(const ModelT<P> &m, const T &a) {
+  const CosmoT<P> &c = m.c;
+  const double e0 = -3.0 * val(1.0 + c.w_DE_0 + c.w_DE_a);
+  T r = pow(a, e0) * exp(3.0 * (a - 1.0) * c.w_DE_a);
+  if constexpr (!std::is_same_v<P, double>) r = r * exp((-3.0 * (1.0 + c.w_DE_0 + c.w_DE_a) - e0) * log(a));
+  return r;
+}
+
+template <class T, class P>
+T aprimeoa(const ModelT<P> &m, const T &a) {
+  const CosmoT<P> &c = m.c;
   T rhonu = exp(spline_eval(m.sp[DEB_SPL_LOGRHONU_LOGA], log(a)));
-  T rhoQ = pow(a, -3.0 * (1.0 + c.w_DE_0 + c.w_DE_a)) * exp(3.0 * (a - 1.0) * c.w_DE_a);
+  T rhoQ = rho_de(m, a);
   T grho = c.grhom * c.Omegam / a + (c.grhog + c.grhor * (c.Neff + c.Nmnu * rhonu)) / (a * a) +
            c.grhom * c.OmegaDE * rhoQ * a * a + c.grhom * c.Omegak;
   return sqrt(grho / 3.0);
 }
 
 // model_synchronous: f = dy/dtau
-template <typename T>
-void rhs(const Model &m, T tau, const T *y, T *f) {
-  const deb_cosmo_t &c = m.c;
+template <class T, class P>
+void rhs(const ModelT<P> &m, const T &tau, const T *y, T *f) {
+  const CosmoT<P> &c = m.c;
   const int lg = m.lmaxg, lgp = m.lmaxgp, lr = m.lmaxr, lnu = m.lmaxnu, nq = m.nq;
   const int n = m.nvar();
   const double k = m.k, k2 = k * k;
-  const double Omegac = c.Omegam - c.Omegab;
+  const P Omegac = c.Omegam - c.Omegab;
   const int iq0 = 10 + lg + lgp + lr;
   const int iq1 = iq0 + nq, iq2 = iq1 + nq, iq3 = iq2 + nq;
   T a = y[0];
   T loga = log(a);
   T eta = y[2], deltac = y[3], thetac = y[4], deltab = y[5], thetab = y[6];
-  T deltag = y[7], thetag = y[8], shearg = y[9] / 2.0;
-  T deltar = y[9 + lg + lgp], thetar = y[10 + lg + lgp], shearr = y[11 + lg + lgp] / 2.0;
-  (void)shearg;
-  (void)shearr;
+  T deltag = y[7], thetag = y[8];
+  T deltar = y[9 + lg + lgp], thetar = y[10 + lg + lgp];
   T deltaq = y[n - 2], thetaq = y[n - 1];
   T cs2 = spline_eval(m.sp[DEB_SPL_CS2A_LOGA], loga) / a;
   T xe = spline_eval(m.sp[DEB_SPL_XE_LOGA], loga);
   T photbar = c.grhog / (c.grhom * c.Omegab * a);
   T pb43 = 4.0 / 3.0 * photbar;
-  double cs2_Q = c.cs2_DE;
+  const P cs2_Q = c.cs2_DE;
   T w_Q = c.w_DE_0 + c.w_DE_a * (1.0 - a);
-  T rho_Q = pow(a, -3.0 * (1.0 + c.w_DE_0 + c.w_DE_a)) * exp(3.0 * (a - 1.0) * c.w_DE_a);
+  T rho_Q = rho_de(m, a);
   T rppQ = (1.0 + w_Q) * rho_Q * c.grhom * c.OmegaDE * thetaq * a * a;
   T aH = aprimeoa(m, a);
   T w_Q_prime = -c.w_DE_a * aH * a;
   T ca2_Q = w_Q - w_Q_prime / 3.0 / ((1.0 + w_Q) + 1e-6) / aH;
-  double akthom = 2.3038921003709498e-9 * (1.0 - c.YHe) * c.Omegab * c.H0 * c.H0;
+  const P akthom = 2.3038921003709498e-9 * (1.0 - c.YHe) * c.Omegab * c.H0 * c.H0;
   T opac = xe * akthom / (a * a);
   f[0] = aH * a;
   // nu_perturb
@@ -224,7 +295,7 @@ void rhs(const Model &m, T tau, const T *y, T *f) {
   f[n - 2] = -(1.0 + w_Q) * (thetaq + 0.5 * hprime) - 3.0 * (cs2_Q - w_Q) * aH * deltaq -
              9.0 * (1.0 + w_Q) * (cs2_Q - ca2_Q) * aH * aH / k2 * thetaq;
   f[n - 1] = -(1.0 - 3.0 * cs2_Q) * aH * thetaq + cs2_Q / (1.0 + w_Q) * k2 * deltaq;
-  if (c.w_DE_0 == -1.0 && c.w_DE_a == 0.0) {
+  if (m.lambda) {
     // cosmological constant: the fluid equations are singular at w = -1
     // (inf * 0 in theta_Q'); their w -> -1 limit keeps delta_Q = theta_Q = 0
     f[n - 2] = T(0.0);
@@ -232,87 +303,88 @@ void rhs(const Model &m, T tau, const T *y, T *f) {
   }
 }
 
-void outputs(const Model &m, const double *y, double *out) {
-  const deb_cosmo_t &c = m.c;
+template <class T, class P>
+void outputs(const ModelT<P> &m, const T *y, T *out) {
+  const CosmoT<P> &c = m.c;
   const int lg = m.lmaxg, lgp = m.lmaxgp, lr = m.lmaxr, nq = m.nq;
   const int n = m.nvar();
   const double k = m.k, k2 = k * k;
-  const double Omegac = c.Omegam - c.Omegab;
+  const P Omegac = c.Omegam - c.Omegab;
   const int iq0 = 10 + lg + lgp + lr, iq1 = iq0 + nq;
-  double a = y[0], eta = y[2], deltac = y[3], thetac = y[4], deltab = y[5], thetab = y[6];
-  double deltag = y[7], thetag = y[8];
-  double deltar = y[9 + lg + lgp], thetar = y[10 + lg + lgp];
-  double rhonu = std::exp(spline_eval(m.sp[DEB_SPL_LOGRHONU_LOGA], std::log(a)));
-  double pnu = std::exp(spline_eval(m.sp[DEB_SPL_LOGPNU_LOGA], std::log(a)));
-  double drhonu = 0, fnu = 0;
+  T a = y[0], eta = y[2], deltac = y[3], thetac = y[4], deltab = y[5], thetab = y[6];
+  T deltag = y[7], thetag = y[8];
+  T deltar = y[9 + lg + lgp], thetar = y[10 + lg + lgp];
+  T rhonu = exp(spline_eval(m.sp[DEB_SPL_LOGRHONU_LOGA], log(a)));
+  T pnu = exp(spline_eval(m.sp[DEB_SPL_LOGPNU_LOGA], log(a)));
+  T drhonu = 0.0, fnu = 0.0;
   for (int i = 0; i < nq; ++i) {
-    double aq = a * c.amnu / m.q[i];
-    double v = 1.0 / std::sqrt(1.0 + aq * aq);
-    drhonu += m.w[i] * y[iq0 + i] / v;
-    fnu += m.w[i] * y[iq1 + i];
+    T aq = a * c.amnu / m.q[i];
+    T v = 1.0 / sqrt(1.0 + aq * aq);
+    drhonu = drhonu + m.w[i] * y[iq0 + i] / v;
+    fnu = fnu + m.w[i] * y[iq1 + i];
   }
-  double deltanu = drhonu / rhonu;
-  double thetanu = k * fnu / (rhonu + pnu);
-  double deltaq = y[n - 2], thetaq = y[n - 1];
-  double w_Q = c.w_DE_0 + c.w_DE_a * (1.0 - a);
-  double rho_Q = std::pow(a, -3.0 * (1.0 + c.w_DE_0 + c.w_DE_a)) * std::exp(3.0 * (a - 1.0) * c.w_DE_a);
-  double rppQ = (1.0 + w_Q) * rho_Q * c.grhom * c.OmegaDE * thetaq * a * a;
-  double a2 = a * a;
-  double grho = c.grhom * c.Omegam / a + (c.grhog + c.grhor * (c.Neff + c.Nmnu * rhonu)) / a2 +
-                c.grhom * c.OmegaDE * rho_Q * a2 + c.grhom * c.Omegak;
-  double aH = std::sqrt(grho / 3.0);
-  double dgrho = c.grhom * (Omegac * deltac + c.Omegab * deltab) / a +
-                 (c.grhog * deltag + c.grhor * (c.Neff * deltar + c.Nmnu * drhonu)) / a2 +
-                 c.grhom * c.OmegaDE * deltaq * rho_Q * a2;
-  double dgtheta = c.grhom * (Omegac * thetac + c.Omegab * thetab) / a +
-                   4.0 / 3.0 * (c.grhog * thetag + c.Neff * c.grhor * thetar) / a2 + c.Nmnu * c.grhor * k * fnu / a2 +
-                   rppQ;
-  double hprime = (2.0 * k2 * eta + dgrho) / aH;
-  double etaprime = 0.5 * dgtheta / k2;
-  double alpha = (hprime + 6.0 * etaprime) / 2.0 / k2;
-  double mden = c.grhom * c.Omegam / a + c.grhor * c.Nmnu * rhonu / a2;
-  double deltam = (c.grhom * (Omegac * deltac + c.Omegab * deltab) / a + c.grhor * c.Nmnu * drhonu / a2) / mden;
-  double thetam = (c.grhom * (Omegac * thetac + c.Omegab * thetab) / a + c.Nmnu * c.grhor * k * fnu / a2) / (3.0 * mden);
-  double deltabc = (c.grhom * (Omegac * deltac + c.Omegab * deltab) / a) / (c.grhom * c.Omegam / a);
-  double thetabc = (c.grhom * (Omegac * thetac + c.Omegab * thetab) / a) / (3.0 * (c.grhom * c.Omegam / a) / a2);
-  thetam += alpha * k2;
-  thetabc += alpha * k2;
-  const double o[20] = {eta, etaprime, hprime, alpha, deltam, thetam / aH, deltabc, thetabc / aH,
-                        deltac, thetac / aH, deltab, thetab / aH, deltag, thetag / aH, deltar, thetar / aH,
-                        deltanu, thetanu / aH, deltaq, thetaq / aH};
-  std::memcpy(out, o, sizeof(o));
+  T deltanu = drhonu / rhonu;
+  T thetanu = k * fnu / (rhonu + pnu);
+  T deltaq = y[n - 2], thetaq = y[n - 1];
+  T w_Q = c.w_DE_0 + c.w_DE_a * (1.0 - a);
+  T rho_Q = rho_de(m, a);
+  T rppQ = (1.0 + w_Q) * rho_Q * c.grhom * c.OmegaDE * thetaq * a * a;
+  T a2 = a * a;
+  T grho = c.grhom * c.Omegam / a + (c.grhog + c.grhor * (c.Neff + c.Nmnu * rhonu)) / a2 +
+           c.grhom * c.OmegaDE * rho_Q * a2 + c.grhom * c.Omegak;
+  T aH = sqrt(grho / 3.0);
+  T dgrho = c.grhom * (Omegac * deltac + c.Omegab * deltab) / a +
+            (c.grhog * deltag + c.grhor * (c.Neff * deltar + c.Nmnu * drhonu)) / a2 +
+            c.grhom * c.OmegaDE * deltaq * rho_Q * a2;
+  T dgtheta = c.grhom * (Omegac * thetac + c.Omegab * thetab) / a +
+              4.0 / 3.0 * (c.grhog * thetag + c.Neff * c.grhor * thetar) / a2 + c.Nmnu * c.grhor * k * fnu / a2 + rppQ;
+  T hprime = (2.0 * k2 * eta + dgrho) / aH;
+  T etaprime = 0.5 * dgtheta / k2;
+  T alpha = (hprime + 6.0 * etaprime) / 2.0 / k2;
+  T mden = c.grhom * c.Omegam / a + c.grhor * c.Nmnu * rhonu / a2;
+  T deltam = (c.grhom * (Omegac * deltac + c.Omegab * deltab) / a + c.grhor * c.Nmnu * drhonu / a2) / mden;
+  T thetam = (c.grhom * (Omegac * thetac + c.Omegab * thetab) / a + c.Nmnu * c.grhor * k * fnu / a2) / (3.0 * mden);
+  T deltabc = (c.grhom * (Omegac * deltac + c.Omegab * deltab) / a) / (c.grhom * c.Omegam / a);
+  T thetabc = (c.grhom * (Omegac * thetac + c.Omegab * thetab) / a) / (3.0 * (c.grhom * c.Omegam / a) / a2);
+  thetam = thetam + alpha * k2;
+  thetabc = thetabc + alpha * k2;
+  const T o[20] = {eta, etaprime, hprime, alpha, deltam, thetam / aH, deltabc, thetabc / aH,
+                   deltac, thetac / aH, deltab, thetab / aH, deltag, thetag / aH, deltar, thetar / aH,
+                   deltanu, thetanu / aH, deltaq, thetaq / aH};
+  for (int i = 0; i < 20; ++i) out[i] = o[i];
 }
 
-void ics(const Model &m, double tau, double *y) {
-  const deb_cosmo_t &c = m.c;
+template <class P>
+void ics(const ModelT<P> &m, const P &tau, P *y) {
+  const CosmoT<P> &c = m.c;
   const int lg = m.lmaxg, lgp = m.lmaxgp, lr = m.lmaxr, nq = m.nq;
   const int n = m.nvar();
   const int iq0 = 10 + lg + lgp + lr, iq1 = iq0 + nq, iq2 = iq1 + nq;
   const double k = m.k;
-  std::fill(y, y + n, 0.0);
-  double a = spline_eval(m.sp[DEB_SPL_A_TAU], tau);
-  double rnu = std::exp(spline_eval(m.sp[DEB_SPL_LOGRHONU_LOGA], std::log(a)));
-  double rhom = c.grhom * c.Omegam / (a * a * a);
-  double rhor = (c.grhog + c.grhor * (c.Neff + c.Nmnu * rnu)) / std::pow(a, 4);
-  double rhonu = c.grhor * (c.Neff + c.Nmnu * rnu) / std::pow(a, 4);
-  double fracb = c.Omegab / c.Omegam;
-  double fracnu = rhonu / rhor;
-  double om = a * rhom / std::sqrt(rhor);
+  for (int i = 0; i < n; ++i) y[i] = P(0.0);
+  P a = spline_eval(m.sp[DEB_SPL_A_TAU], tau);
+  P rnu = exp(spline_eval(m.sp[DEB_SPL_LOGRHONU_LOGA], log(a)));
+  P rhom = c.grhom * c.Omegam / (a * a * a);
+  P rhor = (c.grhog + c.grhor * (c.Neff + c.Nmnu * rnu)) / pow(a, 4.0);
+  P rhonu = c.grhor * (c.Neff + c.Nmnu * rnu) / pow(a, 4.0);
+  P fracb = c.Omegab / c.Omegam;
+  P fracnu = rhonu / rhor;
+  P om = a * rhom / sqrt(rhor);
   const double C = -1.0, s2 = 1.0;
-  double kt = k * tau;
-  double deltag = -(kt * kt) / 3.0 * (1.0 - om * tau / 5.0) * C * s2;
-  double thetag = -(kt * kt * kt) / tau / 36.0 * (1.0 - 3.0 * (1.0 + 5.0 * fracb - fracnu) / 20.0 / (1.0 - fracnu) * om * tau) * C * s2;
-  double deltab = 0.75 * deltag, thetab = thetag, deltac = 0.75 * deltag, thetac = 0.0;
-  double deltar = deltag;
-  double thetar = -(kt * kt * kt * kt) / tau / 36.0 / (4.0 * fracnu + 15.0) *
-                  (4.0 * fracnu + 11.0 + 12.0 - 3.0 * (8.0 * fracnu * fracnu + 50.0 * fracnu + 275.0) / 20.0 / (2.0 * fracnu + 15.0) * tau * om) * C;
-  double shearr = (kt * kt) / (45.0 + 12.0 * fracnu) * (3.0 * s2 - 1.0) * (1.0 + (4.0 * fracnu - 5.0) / 4.0 / (2.0 * fracnu + 15.0) * tau * om) * C;
-  double cs2_Q = c.cs2_DE;
-  double w_Q = c.w_DE_0 + c.w_DE_a * (1.0 - a);
-  double deltaq = (kt * kt) / 4.0 * (1.0 + w_Q) * (4.0 - 3.0 * cs2_Q) / (4.0 - 6.0 * w_Q + 3.0 * cs2_Q) * C * s2;
-  double thetaq = (kt * kt * kt * kt) / tau / 4.0 * cs2_Q / (4.0 - 6.0 * w_Q + 3.0 * cs2_Q) * C * s2;
-  double eta = C * (1.0 - (kt * kt) / 12.0 / (15.0 + 4.0 * fracnu) *
-                              (5.0 + 4.0 * s2 * fracnu - (16.0 * fracnu * fracnu + 280.0 * fracnu + 325.0) / 10.0 / (2.0 * fracnu + 15.0) * tau * om));
+  P kt = k * tau;
+  P deltag = -(kt * kt) / 3.0 * (1.0 - om * tau / 5.0) * C * s2;
+  P thetag = -(kt * kt * kt) / tau / 36.0 * (1.0 - 3.0 * (1.0 + 5.0 * fracb - fracnu) / 20.0 / (1.0 - fracnu) * om * tau) * C * s2;
+  P deltab = 0.75 * deltag, thetab = thetag, deltac = 0.75 * deltag, thetac = 0.0;
+  P deltar = deltag;
+  P thetar = -(kt * kt * kt * kt) / tau / 36.0 / (4.0 * fracnu + 15.0) *
+             (4.0 * fracnu + 11.0 + 12.0 - 3.0 * (8.0 * fracnu * fracnu + 50.0 * fracnu + 275.0) / 20.0 / (2.0 * fracnu + 15.0) * tau * om) * C;
+  P shearr = (kt * kt) / (45.0 + 12.0 * fracnu) * (3.0 * s2 - 1.0) * (1.0 + (4.0 * fracnu - 5.0) / 4.0 / (2.0 * fracnu + 15.0) * tau * om) * C;
+  const P cs2_Q = c.cs2_DE;
+  P w_Q = c.w_DE_0 + c.w_DE_a * (1.0 - a);
+  P deltaq = (kt * kt) / 4.0 * (1.0 + w_Q) * (4.0 - 3.0 * cs2_Q) / (4.0 - 6.0 * w_Q + 3.0 * cs2_Q) * C * s2;
+  P thetaq = (kt * kt * kt * kt) / tau / 4.0 * cs2_Q / (4.0 - 6.0 * w_Q + 3.0 * cs2_Q) * C * s2;
+  P eta = C * (1.0 - (kt * kt) / 12.0 / (15.0 + 4.0 * fracnu) *
+                         (5.0 + 4.0 * s2 * fracnu - (16.0 * fracnu * fracnu + 280.0 * fracnu + 325.0) / 10.0 / (2.0 * fracnu + 15.0) * tau * om));
   y[0] = a;
   y[1] = 0.0;
   y[2] = eta;
@@ -327,8 +399,8 @@ void ics(const Model &m, double tau, double *y) {
   y[11 + lg + lgp] = shearr * 2.0;
   for (int i = 0; i < nq; ++i) {
     double qi = m.q[i];
-    double aq = a * c.amnu / qi;
-    double v = 1.0 / std::sqrt(1.0 + aq * aq);
+    P aq = a * c.amnu / qi;
+    P v = 1.0 / sqrt(1.0 + aq * aq);
     double dlfdlq = -qi / (1.0 + std::exp(-qi));
     y[iq0 + i] = -0.25 * dlfdlq * deltar;
     y[iq1 + i] = -dlfdlq * thetar / v / k / 3.0;
@@ -336,44 +408,48 @@ void ics(const Model &m, double tau, double *y) {
   }
   y[n - 2] = deltaq;
   y[n - 1] = thetaq;
-  if (c.w_DE_0 == -1.0 && c.w_DE_a == 0.0) {
+  if (m.lambda) {
     y[n - 2] = 0.0;
     y[n - 1] = 0.0;
   }
 }
 
-double start_time(const Model &m, double tau_out_min) {
-  const deb_cosmo_t &c = m.c;
-  double tau0 = c.taumin;
-  double tau1 = spline_eval(m.sp[DEB_SPL_TAU_A], 0.1);
+template <class P>
+P start_time(const ModelT<P> &m, const P &tau_out_min) {
+  const CosmoT<P> &c = m.c;
+  P tau0 = c.taumin;
+  P tau1 = spline_eval(m.sp[DEB_SPL_TAU_A], P(0.1));
   double tau_k = 1.0 / m.k;
-  double akthom = 2.3048e-9 * (1.0 - c.YHe) * c.Omegab * c.H0 * c.H0;
-  auto small = [&](double lt) {
-    double tau = std::exp(lt);
-    double xe = spline_eval(m.sp[DEB_SPL_XE_TAU], tau);
-    double a = spline_eval(m.sp[DEB_SPL_A_TAU], tau);
-    double opac = xe * akthom / (a * a);
-    double aH = aprimeoa(m, a);
-    return (1.0 / opac) / (1.0 / aH) / 0.0004 - 1.0;
+  P akthom = 2.3048e-9 * (1.0 - c.YHe) * c.Omegab * c.H0 * c.H0;
+  // the bisections below only compare signs, so the tangent of the root
+  // flows through the bracket end points
+  auto small = [&](const P &lt) {
+    P tau = exp(lt);
+    P xe = spline_eval(m.sp[DEB_SPL_XE_TAU], tau);
+    P a = spline_eval(m.sp[DEB_SPL_A_TAU], tau);
+    P opac = xe * akthom / (a * a);
+    P aH = aprimeoa(m, a);
+    return val((1.0 / opac) / (1.0 / aH) / 0.0004 - 1.0);
   };
-  auto large = [&](double lt) {
-    double tau = std::exp(lt);
-    double a = spline_eval(m.sp[DEB_SPL_A_TAU], tau);
-    double aH = aprimeoa(m, a);
-    return (1.0 / aH) / tau_k / 0.07 - 1.0;
+  auto large = [&](const P &lt) {
+    P tau = exp(lt);
+    P a = spline_eval(m.sp[DEB_SPL_A_TAU], tau);
+    P aH = aprimeoa(m, a);
+    return val((1.0 / aH) / tau_k / 0.07 - 1.0);
   };
   auto bisect = [&](auto fn) {
-    double xl = std::log(tau0), xr = std::log(tau1);
+    P xl = log(tau0), xr = log(tau1);
     for (int i = 0; i < 7; ++i) {
-      double xm = 0.5 * (xl + xr);
+      P xm = 0.5 * (xl + xr);
       if (fn(xm) * fn(xl) > 0) xl = xm; else xr = xm;
     }
-    return 0.5 * (xl + xr);
+    return P(0.5 * (xl + xr));
   };
-  double lt_large = bisect(large);
-  double lt_small = bisect(small);
-  double ts = std::exp(std::min(lt_small, lt_large));
-  return 0.99 * std::min(tau_out_min, ts);
+  P lt_large = bisect(large);
+  P lt_small = bisect(small);
+  P ts = exp(val(lt_small) < val(lt_large) ? lt_small : lt_large);
+  P tmin = val(tau_out_min) < val(ts) ? tau_out_min : ts;
+  return 0.99 * tmin;
 }
 
 // ---------------------------------------------------------------------------
@@ -396,79 +472,95 @@ const double d1 = gamma, d2 = -0.1823079225333714636, d3 = -0.319231832186874912
              d5 = -0.377417564392089818;
 }  // namespace r5
 
+
+template <class T>
 struct Work {
   int n;
-  std::vector<double> J, LU, dT, f, u, rhs, yt;
+  std::vector<T> J, LU, dT, f, u, rhs;
   std::vector<int> piv;
-  std::vector<double> K[8];
-  std::vector<Dual> yd, fd;
-  explicit Work(int n_) : n(n_), J(n_ * n_), LU(n_ * n_), dT(n_), f(n_), u(n_), rhs(n_), yt(n_), piv(n_), yd(n_), fd(n_) {
+  std::vector<T> K[8];
+  std::vector<D<T>> yd, fd;
+  explicit Work(int n_) : n(n_), J(n_ * n_), LU(n_ * n_), dT(n_), f(n_), u(n_), rhs(n_), piv(n_), yd(n_), fd(n_) {
     for (auto &v : K) v.resize(n_);
   }
 };
 
-void jacobian(const Model &m, double t, const double *y, Work &w) {
+// exact J = df/dy and dT = df/dtau by forward-mode duals (jax.jacfwd)
+template <class T, class P>
+void jacobian(const ModelT<P> &m, const T &t, const T *y, Work<T> &w) {
   const int n = w.n;
   for (int j = 0; j < n; ++j) {
-    for (int i = 0; i < n; ++i) w.yd[i] = Dual(y[i], i == j ? 1.0 : 0.0);
-    rhs<Dual>(m, Dual(t, 0.0), w.yd.data(), w.fd.data());
+    for (int i = 0; i < n; ++i) w.yd[i] = D<T>(y[i], T(i == j ? 1.0 : 0.0));
+    rhs<D<T>, P>(m, D<T>(t), w.yd.data(), w.fd.data());
     for (int i = 0; i < n; ++i) w.J[i * n + j] = w.fd[i].d;
   }
-  for (int i = 0; i < n; ++i) w.yd[i] = Dual(y[i], 0.0);
-  rhs<Dual>(m, Dual(t, 1.0), w.yd.data(), w.fd.data());
+  for (int i = 0; i < n; ++i) w.yd[i] = D<T>(y[i]);
+  rhs<D<T>, P>(m, D<T>(t, T(1.0)), w.yd.data(), w.fd.data());
   for (int i = 0; i < n; ++i) w.dT[i] = w.fd[i].d;
 }
 
 // LAPACK getrf-style partial pivoting (first max |.| in the column)
-void lu_factor(double *A, int *piv, int n) {
+template <class T>
+void lu_factor(T *A, int *piv, int n) {
   for (int kcol = 0; kcol < n; ++kcol) {
     int p = kcol;
-    double best = std::fabs(A[kcol * n + kcol]);
+    double best = std::fabs(val(A[kcol * n + kcol]));
     for (int i = kcol + 1; i < n; ++i) {
-      double v = std::fabs(A[i * n + kcol]);
+      double v = std::fabs(val(A[i * n + kcol]));
       if (v > best) { best = v; p = i; }
     }
     piv[kcol] = p;
     if (p != kcol)
       for (int j = 0; j < n; ++j) std::swap(A[kcol * n + j], A[p * n + j]);
-    double d = A[kcol * n + kcol];
-    if (d != 0.0) {
-      double inv = 1.0 / d;
+    T d = A[kcol * n + kcol];
+    if (val(d) != 0.0) {
+      T inv = 1.0 / d;
       for (int i = kcol + 1; i < n; ++i) {
-        double l = A[i * n + kcol] * inv;
-        A[i * n + kcol] = l;
-        if (l != 0.0) {
-          const double *rk = A + kcol * n;
-          double *ri = A + i * n;
-          for (int j = kcol + 1; j < n; ++j) ri[j] -= l * rk[j];
+        if (val(A[i * n + kcol]) == 0.0 && val(A[i * n + kcol] * inv) == 0.0) {
+          // structurally zero entry: keep the (possibly nonzero) tangent exact
+          T l = A[i * n + kcol] * inv;
+          A[i * n + kcol] = l;
+          const T *rk = A + kcol * n;
+          T *ri = A + i * n;
+          for (int j = kcol + 1; j < n; ++j) ri[j] = ri[j] - l * rk[j];
+          continue;
         }
+        T l = A[i * n + kcol] * inv;
+        A[i * n + kcol] = l;
+        const T *rk = A + kcol * n;
+        T *ri = A + i * n;
+        for (int j = kcol + 1; j < n; ++j) ri[j] = ri[j] - l * rk[j];
       }
     }
   }
 }
 
-void lu_solve(const double *LU, const int *piv, int n, double *b) {
+template <class T>
+void lu_solve(const T *LU, const int *piv, int n, T *b) {
   for (int i = 0; i < n; ++i)
     if (piv[i] != i) std::swap(b[i], b[piv[i]]);
   for (int i = 1; i < n; ++i) {
-    double s = b[i];
-    const double *r = LU + i * n;
-    for (int j = 0; j < i; ++j) s -= r[j] * b[j];
+    T s = b[i];
+    const T *r = LU + i * n;
+    for (int j = 0; j < i; ++j) s = s - r[j] * b[j];
     b[i] = s;
   }
   for (int i = n - 1; i >= 0; --i) {
-    double s = b[i];
-    const double *r = LU + i * n;
-    for (int j = i + 1; j < n; ++j) s -= r[j] * b[j];
+    T s = b[i];
+    const T *r = LU + i * n;
+    for (int j = i + 1; j < n; ++j) s = s - r[j] * b[j];
     b[i] = s / r[i];
   }
 }
 
 // problem adaptor: the perturbation system of one mode
+template <class P>
 struct ModeSystem {
-  const Model &m;
-  void f(double t, const double *y, double *out) const { rhs<double>(m, t, y, out); }
-  void jac(double t, const double *y, Work &w) const { jacobian(m, t, y, w); }
+  const ModelT<P> &m;
+  template <class T>
+  void f(const T &t, const T *y, T *out) const { rhs<T, P>(m, t, y, out); }
+  template <class T>
+  void jac(const T &t, const T *y, Work<T> &w) const { jacobian<T, P>(m, t, y, w); }
 };
 
 // linear test system f = A y + g0 + g1 t (pins the step on closed forms)
@@ -482,26 +574,26 @@ struct LinearSystem {
       out[i] = s;
     }
   }
-  void jac(double, const double *, Work &w) const {
+  void jac(double, const double *, Work<double> &w) const {
     for (int i = 0; i < n * n; ++i) w.J[i] = A[i];
     for (int i = 0; i < n; ++i) w.dT[i] = g1[i];
   }
 };
 
 // one Rodas5Transformed step; returns y1 in y1, error estimate (k8) in err
-template <typename Sys>
-void rodas5_step_sys(const Sys &sys, double t0, double t1, const double *y0, double *y1, double *err, Work &w) {
+template <class T, class Sys>
+void rodas5_step_sys(const Sys &sys, const T &t0, const T &t1, const T *y0, T *y1, T *err, Work<T> &w) {
   using namespace r5;
   const int n = w.n;
-  const double dt = t1 - t0;
+  const T dt = t1 - t0;
   sys.jac(t0, y0, w);
-  const double dtg = dt * gamma;
+  const T dtg = dt * gamma;
   for (int i = 0; i < n * n; ++i) w.LU[i] = -w.J[i];
-  for (int i = 0; i < n; ++i) w.LU[i * n + i] += 1.0 / dtg;
+  for (int i = 0; i < n; ++i) w.LU[i * n + i] = w.LU[i * n + i] + 1.0 / dtg;
   lu_factor(w.LU.data(), w.piv.data(), n);
   auto &k1 = w.K[0], &k2 = w.K[1], &k3 = w.K[2], &k4 = w.K[3], &k5 = w.K[4], &k6 = w.K[5], &k7 = w.K[6], &k8 = w.K[7];
-  auto F = [&](double t, const double *u) { sys.f(t, u, w.f.data()); };
-  auto solve = [&](std::vector<double> &k) {
+  auto F = [&](const T &t, const T *u) { sys.f(t, u, w.f.data()); };
+  auto solve = [&](std::vector<T> &k) {
     std::copy(w.rhs.begin(), w.rhs.end(), k.begin());
     lu_solve(w.LU.data(), w.piv.data(), n, k.data());
   };
@@ -551,8 +643,9 @@ void rodas5_step_sys(const Sys &sys, double t0, double t1, const double *y0, dou
   }
 }
 
-void rodas5_step(const Model &m, double t0, double t1, const double *y0, double *y1, double *err, Work &w) {
-  rodas5_step_sys(ModeSystem{m}, t0, t1, y0, y1, err, w);
+template <class T, class P>
+void rodas5_step(const ModelT<P> &m, const T &t0, const T &t1, const T *y0, T *y1, T *err, Work<T> &w) {
+  rodas5_step_sys<T>(ModeSystem<P>{m}, t0, t1, y0, y1, err, w);
 }
 
 struct SolverCfg {
@@ -561,20 +654,22 @@ struct SolverCfg {
   int return_full;
 };
 
-// evolve_one_mode: returns status; fills out[nout * (return_full ? n : 20)]
+// evolve_one_mode: returns status; fills out[nout * (return_full ? n : 20)].
+// times (optional) receives t0 and the end time of every accepted step.
 int evolve_mode(const Model &m, const SolverCfg &s, const double *tau_out, int nout, double *out, int *nsteps,
-                int *naccept, double *tau_start_out) {
+                int *naccept, double *tau_start_out, std::vector<double> *times = nullptr) {
   const int n = m.nvar();
-  Work w(n);
+  Work<double> w(n);
   double tmin = tau_out[0];
   for (int i = 1; i < nout; ++i) tmin = std::min(tmin, tau_out[i]);
   double tmax = tau_out[0];
   for (int i = 1; i < nout; ++i) tmax = std::max(tmax, tau_out[i]);
-  const double t0 = start_time(m, tmin);
+  const double t0 = start_time<double>(m, tmin);
   if (tau_start_out) *tau_start_out = t0;
+  if (times) times->assign(1, t0);
   const double t1 = tmax;
   std::vector<double> y(n), y1(n), err(n), ys(n);
-  ics(m, t0, y.data());
+  ics<double>(m, t0, y.data());
   const double dt0 = std::min(t0 / 4.0, 0.5 * (t1 - t0));
   double tprev = t0, tnext = std::min(t0 + dt0, t1);
   double prev_inv = 1.0, prev_prev_inv = 1.0;
@@ -589,7 +684,7 @@ int evolve_mode(const Model &m, const SolverCfg &s, const double *tau_out, int n
   int status = DEB_STATUS_OK;
   while (tprev < t1) {
     if (steps >= s.max_steps) { status = DEB_STATUS_MAX_STEPS; break; }
-    rodas5_step(m, tprev, tnext, y.data(), y1.data(), err.data(), w);
+    rodas5_step<double, double>(m, tprev, tnext, y.data(), y1.data(), err.data(), w);
     double ss = 0.0;
     for (int j = 0; j < 6; ++j) {
       int i = fidx[j];
@@ -613,13 +708,14 @@ int evolve_mode(const Model &m, const SolverCfg &s, const double *tau_out, int n
         double th = (tau_out[iout] - tprev) / (tnext - tprev);
         for (int i = 0; i < n; ++i) ys[i] = y[i] + th * (y1[i] - y[i]);
         if (s.return_full) std::copy(ys.begin(), ys.end(), out + (size_t)iout * width);
-        else outputs(m, ys.data(), out + (size_t)iout * width);
+        else outputs<double, double>(m, ys.data(), out + (size_t)iout * width);
         ++iout;
       }
       prev_prev_inv = prev_inv;
       prev_inv = inv;
       y.swap(y1);
       ++acc;
+      if (times) times->push_back(tnext);
     }
     double nt0 = keep ? tnext : tprev;
     double nt1 = nt0 + dt;
@@ -633,14 +729,59 @@ int evolve_mode(const Model &m, const SolverCfg &s, const double *tau_out, int n
   return status;
 }
 
-Model make_model(const deb_cosmo_t *c, const double *tables, const int32_t *desc, const deb_config_t *cfg,
-                 const double *q, const double *w, double k) {
-  Model m;
-  m.c = *c;
-  for (int s = 0; s < DEB_NSPLINE; ++s) {
-    int off = desc[2 * s], n = desc[2 * s + 1];
-    m.sp[s] = {tables + off, tables + off + n, tables + off + 2 * n, n};
+// directional derivative of the 20 output fields along one tangent of the
+// inputs, with the accepted steps of the plain solve held fixed as fractions
+// of [t0, t1] (see the file header)
+int evolve_tangent(const Model &m, const ModelT<Dd> &md, const SolverCfg &s, const double *tau_out,
+                   const double *dtau_out, int nout, double *out, double *dout, int *naccept) {
+  const int n = m.nvar();
+  std::vector<double> times, plain((size_t)nout * 20);
+  int nsteps = 0, nacc = 0;
+  double ts0 = 0.0;
+  SolverCfg s20 = s;
+  s20.return_full = 0;
+  int st = evolve_mode(m, s20, tau_out, nout, plain.data(), &nsteps, &nacc, &ts0, &times);
+  *naccept = nacc;
+  if (st != DEB_STATUS_OK) return st;
+  std::vector<Dd> tout(nout);
+  for (int i = 0; i < nout; ++i) tout[i] = Dd(tau_out[i], dtau_out[i]);
+  Dd tmin = tout[0], tmax = tout[0];
+  for (int i = 1; i < nout; ++i) {
+    if (tau_out[i] < val(tmin)) tmin = tout[i];
+    if (tau_out[i] > val(tmax)) tmax = tout[i];
   }
+  const Dd T0 = start_time<Dd>(md, tmin);
+  const Dd T1 = tmax;
+  const double span = times.back() - times.front();
+  auto time_of = [&](size_t i) {
+    double sfr = (times[i] - times.front()) / span;
+    return Dd(times[i], (1.0 - sfr) * T0.d + sfr * T1.d);
+  };
+  Work<Dd> w(n);
+  std::vector<Dd> y(n), y1(n), err(n), ys(n), o(20);
+  ics<Dd>(md, T0, y.data());
+  int iout = 0;
+  for (size_t st_i = 0; st_i + 1 < times.size(); ++st_i) {
+    const Dd ta = time_of(st_i), tb = time_of(st_i + 1);
+    rodas5_step<Dd, Dd>(md, ta, tb, y.data(), y1.data(), err.data(), w);
+    while (iout < nout && tau_out[iout] <= times[st_i + 1]) {
+      Dd th = (tout[iout] - ta) / (tb - ta);
+      for (int i = 0; i < n; ++i) ys[i] = y[i] + th * (y1[i] - y[i]);
+      outputs<Dd, Dd>(md, ys.data(), o.data());
+      for (int f = 0; f < 20; ++f) {
+        out[(size_t)iout * 20 + f] = o[f].v;
+        dout[(size_t)iout * 20 + f] = o[f].d;
+      }
+      ++iout;
+    }
+    y.swap(y1);
+  }
+  return DEB_STATUS_OK;
+}
+
+template <class P>
+void model_common(ModelT<P> &m, const deb_cosmo_t *c, const deb_config_t *cfg, const double *q, const double *w,
+                  double k) {
   m.k = k;
   m.lmaxg = cfg->lmaxg;
   m.lmaxgp = cfg->lmaxgp;
@@ -651,6 +792,38 @@ Model make_model(const deb_cosmo_t *c, const double *tables, const int32_t *desc
     m.q[i] = q[i];
     m.w[i] = w[i];
   }
+  m.lambda = (c->w_DE_0 == -1.0 && c->w_DE_a == 0.0);
+}
+
+Model make_model(const deb_cosmo_t *c, const double *tables, const int32_t *desc, const deb_config_t *cfg,
+                 const double *q, const double *w, double k) {
+  Model m;
+  m.c = cosmo_of(*c);
+  for (int s = 0; s < DEB_NSPLINE; ++s) {
+    int off = desc[2 * s], n = desc[2 * s + 1];
+    m.sp[s] = {tables + off, tables + off + n, tables + off + 2 * n, n};
+  }
+  model_common(m, c, cfg, q, w, k);
+  return m;
+}
+
+// tangent-carrying model; knots (value, tangent) live in `store`
+ModelT<Dd> make_model_tangent(const deb_cosmo_t *c, const double *tables, const deb_cosmo_t *dc,
+                              const double *dtables, const int32_t *desc, const deb_config_t *cfg, const double *q,
+                              const double *w, double k, std::vector<Dd> &store) {
+  ModelT<Dd> m;
+  m.c = cosmo_of(*c, *dc);
+  size_t tot = 0;
+  for (int s = 0; s < DEB_NSPLINE; ++s) tot += 3 * (size_t)desc[2 * s + 1];
+  store.resize(tot);
+  size_t pos = 0;
+  for (int s = 0; s < DEB_NSPLINE; ++s) {
+    int off = desc[2 * s], n = desc[2 * s + 1];
+    for (int j = 0; j < 3 * n; ++j) store[pos + j] = Dd(tables[off + j], dtables[off + j]);
+    m.sp[s] = {store.data() + pos, store.data() + pos + n, store.data() + pos + 2 * n, n};
+    pos += 3 * (size_t)n;
+  }
+  model_common(m, c, cfg, q, w, k);
   return m;
 }
 
@@ -693,7 +866,7 @@ int oracle_nvar(const deb_config_t *cfg) {
 int oracle_rhs(const deb_config_t *cfg, const deb_cosmo_t *c, const double *tables, const int32_t *desc,
                const double *q, const double *w, double k, double tau, const double *y, double *f) {
   Model m = make_model(c, tables, desc, cfg, q, w, k);
-  rhs<double>(m, tau, y, f);
+  rhs<double, double>(m, tau, y, f);
   return 0;
 }
 
@@ -701,8 +874,8 @@ int oracle_rhs(const deb_config_t *cfg, const deb_cosmo_t *c, const double *tabl
 int oracle_jacobian(const deb_config_t *cfg, const deb_cosmo_t *c, const double *tables, const int32_t *desc,
                     const double *q, const double *w, double k, double tau, const double *y, double *J, double *dT) {
   Model m = make_model(c, tables, desc, cfg, q, w, k);
-  Work wk(m.nvar());
-  jacobian(m, tau, y, wk);
+  Work<double> wk(m.nvar());
+  jacobian<double, double>(m, tau, y, wk);
   std::copy(wk.J.begin(), wk.J.end(), J);
   std::copy(wk.dT.begin(), wk.dT.end(), dT);
   return 0;
@@ -712,24 +885,24 @@ int oracle_rodas5_step(const deb_config_t *cfg, const deb_cosmo_t *c, const doub
                        const double *q, const double *w, double k, double t0, double t1, const double *y0, double *y1,
                        double *err) {
   Model m = make_model(c, tables, desc, cfg, q, w, k);
-  Work wk(m.nvar());
-  rodas5_step(m, t0, t1, y0, y1, err, wk);
+  Work<double> wk(m.nvar());
+  rodas5_step<double, double>(m, t0, t1, y0, y1, err, wk);
   return 0;
 }
 
 int oracle_ics(const deb_config_t *cfg, const deb_cosmo_t *c, const double *tables, const int32_t *desc,
                const double *q, const double *w, double k, double tau_out_min, double *tau_start, double *y0) {
   Model m = make_model(c, tables, desc, cfg, q, w, k);
-  *tau_start = start_time(m, tau_out_min);
-  ics(m, *tau_start, y0);
+  *tau_start = start_time<double>(m, tau_out_min);
+  ics<double>(m, *tau_start, y0);
   return 0;
 }
 
 // one Rodas5 step of the linear system y' = A y + g0 + g1 t (same step code)
 int oracle_rodas5_linear(int n, const double *A, const double *g0, const double *g1, double t0, double t1,
                          const double *y0, double *y1, double *err) {
-  Work wk(n);
-  rodas5_step_sys(LinearSystem{n, A, g0, g1}, t0, t1, y0, y1, err, wk);
+  Work<double> wk(n);
+  rodas5_step_sys<double>(LinearSystem{n, A, g0, g1}, t0, t1, y0, y1, err, wk);
   return 0;
 }
 
@@ -737,14 +910,14 @@ int oracle_rodas5_linear(int n, const double *A, const double *g0, const double 
 int oracle_ics_at(const deb_config_t *cfg, const deb_cosmo_t *c, const double *tables, const int32_t *desc,
                   const double *q, const double *w, double k, double tau, double *y0) {
   Model m = make_model(c, tables, desc, cfg, q, w, k);
-  ics(m, tau, y0);
+  ics<double>(m, tau, y0);
   return 0;
 }
 
 int oracle_outputs(const deb_config_t *cfg, const deb_cosmo_t *c, const double *tables, const int32_t *desc,
                    const double *q, const double *w, double k, const double *y, double *out20) {
   Model m = make_model(c, tables, desc, cfg, q, w, k);
-  outputs(m, y, out20);
+  outputs<double, double>(m, y, out20);
   return 0;
 }
 
@@ -769,6 +942,116 @@ int oracle_evolve(const deb_config_t *cfg, const deb_cosmo_t *c, const double *t
       nsteps[i] = st;
       naccept[i] = na;
       if (tau_start) tau_start[i] = ts;
+    }
+  };
+  if (nthreads <= 1) {
+    worker();
+  } else {
+    std::vector<std::thread> th;
+    for (int t = 0; t < nthreads; ++t) th.emplace_back(worker);
+    for (auto &t : th) t.join();
+  }
+  return 0;
+}
+
+// ---- tangents (directional derivatives along dcosmo, dtables, dtau, dy) ----
+
+// f and its tangent
+int oracle_rhs_tangent(const deb_config_t *cfg, const deb_cosmo_t *c, const double *tables,
+                       const deb_cosmo_t *dc, const double *dtables, const int32_t *desc, const double *q,
+                       const double *w, double k, double tau, double dtau, const double *y, const double *dy,
+                       double *f, double *df) {
+  std::vector<Dd> store;
+  ModelT<Dd> m = make_model_tangent(c, tables, dc, dtables, desc, cfg, q, w, k, store);
+  const int n = m.nvar();
+  std::vector<Dd> yd(n), fd(n);
+  for (int i = 0; i < n; ++i) yd[i] = Dd(y[i], dy[i]);
+  rhs<Dd, Dd>(m, Dd(tau, dtau), yd.data(), fd.data());
+  for (int i = 0; i < n; ++i) {
+    f[i] = fd[i].v;
+    df[i] = fd[i].d;
+  }
+  return 0;
+}
+
+// one Rodas5 step and its tangent along (dt0, dt1, dy0, dcosmo, dtables)
+int oracle_rodas5_step_tangent(const deb_config_t *cfg, const deb_cosmo_t *c, const double *tables,
+                               const deb_cosmo_t *dc, const double *dtables, const int32_t *desc, const double *q,
+                               const double *w, double k, double t0, double dt0, double t1, double dt1,
+                               const double *y0, const double *dy0, double *y1, double *dy1) {
+  std::vector<Dd> store;
+  ModelT<Dd> m = make_model_tangent(c, tables, dc, dtables, desc, cfg, q, w, k, store);
+  const int n = m.nvar();
+  Work<Dd> wk(n);
+  std::vector<Dd> a(n), b(n), e(n);
+  for (int i = 0; i < n; ++i) a[i] = Dd(y0[i], dy0[i]);
+  rodas5_step<Dd, Dd>(m, Dd(t0, dt0), Dd(t1, dt1), a.data(), b.data(), e.data(), wk);
+  for (int i = 0; i < n; ++i) {
+    y1[i] = b[i].v;
+    dy1[i] = b[i].d;
+  }
+  return 0;
+}
+
+// 20 output fields and their tangent along (dy, dcosmo, dtables)
+int oracle_outputs_tangent(const deb_config_t *cfg, const deb_cosmo_t *c, const double *tables,
+                           const deb_cosmo_t *dc, const double *dtables, const int32_t *desc, const double *q,
+                           const double *w, double k, const double *y, const double *dy, double *out20,
+                           double *dout20) {
+  std::vector<Dd> store;
+  ModelT<Dd> m = make_model_tangent(c, tables, dc, dtables, desc, cfg, q, w, k, store);
+  const int n = m.nvar();
+  std::vector<Dd> yd(n), o(20);
+  for (int i = 0; i < n; ++i) yd[i] = Dd(y[i], dy[i]);
+  outputs<Dd, Dd>(m, yd.data(), o.data());
+  for (int f = 0; f < 20; ++f) {
+    out20[f] = o[f].v;
+    dout20[f] = o[f].d;
+  }
+  return 0;
+}
+
+// start time, initial state and their tangents along (dtau_out_min, dcosmo, dtables)
+int oracle_ics_tangent(const deb_config_t *cfg, const deb_cosmo_t *c, const double *tables,
+                       const deb_cosmo_t *dc, const double *dtables, const int32_t *desc, const double *q,
+                       const double *w, double k, double tau_out_min, double dtau_out_min, double *tau_start,
+                       double *dtau_start, double *y0, double *dy0) {
+  std::vector<Dd> store;
+  ModelT<Dd> m = make_model_tangent(c, tables, dc, dtables, desc, cfg, q, w, k, store);
+  const int n = m.nvar();
+  Dd ts = start_time<Dd>(m, Dd(tau_out_min, dtau_out_min));
+  std::vector<Dd> y(n);
+  ics<Dd>(m, ts, y.data());
+  *tau_start = ts.v;
+  *dtau_start = ts.d;
+  for (int i = 0; i < n; ++i) {
+    y0[i] = y[i].v;
+    dy0[i] = y[i].d;
+  }
+  return 0;
+}
+
+// Per-mode solve plus the tangent of its 20 output fields along one input
+// tangent (dcosmo, dtables, dtau_out), threaded over modes.
+int oracle_evolve_tangent(const deb_config_t *cfg, const deb_cosmo_t *c, const double *tables,
+                          const deb_cosmo_t *dc, const double *dtables, const int32_t *desc, const double *q,
+                          const double *w, int nk, const double *kmodes, int nout, const double *tau_out,
+                          const double *dtau_out, double *y_out, double *dy_out, int32_t *status, int32_t *naccept,
+                          int nthreads) {
+  SolverCfg s{cfg->rtol, cfg->atol, cfg->pcoeff, cfg->icoeff, cfg->dcoeff, cfg->factormax, cfg->factormin,
+              cfg->max_steps, 0};
+  std::atomic<int> next(0);
+  auto worker = [&]() {
+    std::vector<Dd> store;
+    for (;;) {
+      int i = next.fetch_add(1);
+      if (i >= nk) break;
+      Model m = make_model(c, tables, desc, cfg, q, w, kmodes[i]);
+      ModelT<Dd> md = make_model_tangent(c, tables, dc, dtables, desc, cfg, q, w, kmodes[i], store);
+      int na = 0;
+      status[i] = evolve_tangent(m, md, s, tau_out, dtau_out, nout, y_out + (size_t)i * nout * 20,
+                                 dy_out + (size_t)i * nout * 20, &na);
+      naccept[i] = na;
     }
   };
   if (nthreads <= 1) {

@@ -117,3 +117,65 @@ class Problem:
         lib().oracle_evolve(*self._common(), ctypes.c_int(nk), _p(kmodes), ctypes.c_int(nout), _p(tau_out), _p(y),
                             _p(st, _ip), _p(ns, _ip), _p(na, _ip), _p(ts), ctypes.c_int(nthreads))
         return dict(y=y, status=st, nsteps=ns, naccept=na, tau_start=ts)
+
+    # ---- tangents along one input direction (dcos[24], dtab[stride]) ----
+    def _tan(self, dcos, dtab):
+        dcos = np.ascontiguousarray(dcos, dtype=np.float64)
+        dtab = np.ascontiguousarray(dtab, dtype=np.float64)
+        self._keep = (dcos, dtab)
+        return (ctypes.byref(self.cfg), _p(self.cosmo), _p(self.tables), _p(dcos), _p(dtab), _p(self.desc, _ip),
+                _p(self.q), _p(self.w))
+
+    def rhs_tangent(self, k, tau, dtau, y, dy, dcos, dtab):
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        dy = np.ascontiguousarray(dy, dtype=np.float64)
+        f, df = np.zeros(self.n), np.zeros(self.n)
+        lib().oracle_rhs_tangent(*self._tan(dcos, dtab), ctypes.c_double(k), ctypes.c_double(tau),
+                                 ctypes.c_double(dtau), _p(y), _p(dy), _p(f), _p(df))
+        return f, df
+
+    def step_tangent(self, k, t0, dt0, t1, dt1, y0, dy0, dcos, dtab):
+        y0 = np.ascontiguousarray(y0, dtype=np.float64)
+        dy0 = np.ascontiguousarray(dy0, dtype=np.float64)
+        y1, dy1 = np.zeros(self.n), np.zeros(self.n)
+        lib().oracle_rodas5_step_tangent(*self._tan(dcos, dtab), ctypes.c_double(k), ctypes.c_double(t0),
+                                         ctypes.c_double(dt0), ctypes.c_double(t1), ctypes.c_double(dt1), _p(y0),
+                                         _p(dy0), _p(y1), _p(dy1))
+        return y1, dy1
+
+    def outputs_tangent(self, k, y, dy, dcos, dtab):
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        dy = np.ascontiguousarray(dy, dtype=np.float64)
+        o, do = np.zeros(20), np.zeros(20)
+        lib().oracle_outputs_tangent(*self._tan(dcos, dtab), ctypes.c_double(k), _p(y), _p(dy), _p(o), _p(do))
+        return o, do
+
+    def ics_tangent(self, k, tmin, dtmin, dcos, dtab):
+        ts, dts = ctypes.c_double(0.0), ctypes.c_double(0.0)
+        y0, dy0 = np.zeros(self.n), np.zeros(self.n)
+        lib().oracle_ics_tangent(*self._tan(dcos, dtab), ctypes.c_double(k), ctypes.c_double(tmin),
+                                 ctypes.c_double(dtmin), ctypes.byref(ts), ctypes.byref(dts), _p(y0), _p(dy0))
+        return ts.value, dts.value, y0, dy0
+
+    def evolve_tangent(self, kmodes, tau_out, dtau_out, dcos, dtab, nthreads: int = 1):
+        """20 output fields and their tangent along (dcos, dtab, dtau_out), with the
+        accepted steps of the plain solve held fixed as fractions of [t0, t1]."""
+        kmodes = np.ascontiguousarray(kmodes, dtype=np.float64)
+        tau_out = np.ascontiguousarray(np.atleast_1d(tau_out), dtype=np.float64)
+        dtau_out = np.ascontiguousarray(np.atleast_1d(dtau_out), dtype=np.float64)
+        nk, nout = kmodes.size, tau_out.size
+        y, dy = np.zeros((nk, nout, 20)), np.zeros((nk, nout, 20))
+        st = np.zeros(nk, dtype=np.int32)
+        na = np.zeros(nk, dtype=np.int32)
+        lib().oracle_evolve_tangent(*self._tan(dcos, dtab), ctypes.c_int(nk), _p(kmodes), ctypes.c_int(nout),
+                                    _p(tau_out), _p(dtau_out), _p(y), _p(dy), _p(st, _ip), _p(na, _ip),
+                                    ctypes.c_int(nthreads))
+        return dict(y=y, dy=dy, status=st, naccept=na)
+
+    def perturbed(self, eps, dcos, dtab):
+        """A copy whose inputs are moved by eps along (dcos, dtab)."""
+        other = Problem.__new__(Problem)
+        other.__dict__.update(self.__dict__)
+        other.cosmo = np.ascontiguousarray(self.cosmo + eps * np.asarray(dcos))
+        other.tables = np.ascontiguousarray(self.tables + eps * np.asarray(dtab))
+        return other

@@ -51,3 +51,17 @@ def gpu():
     from discoeb_amd import _lib
     _lib.lib()
     return torch.device("cuda:0")
+
+
+@pytest.fixture(scope="session")
+def grad_inputs():
+    """w0waCDM + massive-nu base cosmology with its input tangents along the six
+    theta directions that enter the perturbation solve (complex step)."""
+    from discoeb_amd import tangents
+    from discoeb_amd.cosmology import base_theta
+    import warnings
+    th = base_theta()[None]
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        param, dcos, dtab, dtau = tangents.input_tangents(th, [0.5, 1.0])
+    return dict(theta=th, param=param, dcos=dcos, dtab=dtab, dtau=dtau, aexp_out=[0.5, 1.0])
