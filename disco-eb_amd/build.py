@@ -12,8 +12,11 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "discoeb_amd", "libdiscoeb_hip.so")
-SOURCES = ["deb_kernels.hip"]
-HEADERS = ["deb_device.h", "deb_mode.h", "deb_solver.h"]
+SOURCES = ["deb_kernels.hip", "deb_adjoint.hip"]
+HEADERS = ["deb_device.h", "deb_mode.h", "deb_solver.h", "deb_forward.h", "deb_host.h", "deb_adjoint.h"]
+FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         "-Wno-unused-command-line-argument"]
 
 
 def hipcc() -> str:
@@ -24,17 +27,35 @@ def hipcc() -> str:
 
 
 def build(verbose: bool = False) -> str:
-    cmd = [hipcc(), "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
-           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-           "-Wno-unused-command-line-argument",
-           "-I", os.path.join(HERE, "..", "include"),
-           "-o", OUT + ".tmp"] + [os.path.join(CSRC, s) for s in SOURCES]
+    """Compile every translation unit in parallel, then link the shared library."""
+    inc = ["-I", os.path.join(HERE, "..", "include")]
+    if os.environ.get("DEB_NS_MASK"):  # development builds: fewer slot counts
+        inc += ["-DDEB_NS_MASK=" + os.environ["DEB_NS_MASK"]]
+    objs, procs = [], []
+    for src in SOURCES:
+        obj = os.path.join(CSRC, "_" + src.replace(".hip", ".o"))
+        cmd = [hipcc()] + FLAGS + inc + ["-c", "-o", obj, os.path.join(CSRC, src)]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
+        objs.append(obj)
+    failed = False
+    for src, p in procs:
+        out, _ = p.communicate()
+        if p.returncode != 0:
+            sys.stderr.write(out)
+            failed = True
+    if failed:
+        raise RuntimeError("hipcc failed")
+    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", OUT + ".tmp"] + objs
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed")
+        raise RuntimeError("hipcc link failed")
+    for o in objs:
+        os.remove(o)
     os.replace(OUT + ".tmp", OUT)
     return OUT
 

@@ -1,0 +1,752 @@
+// Reverse-mode (adjoint) building blocks of the gradient path.
+//
+// The reference differentiates evolve_perturbations with jax.jacfwd
+// (nb_minimal_example.ipynb:297) straight through diffrax's loop and
+// Rodas5Transformed.step (ode_integrators_stiff.py:718-834).  Here the same
+// derivative of the discrete solution is accumulated backwards, one
+// wavefront per mode: the accepted states come from the checkpoint tape,
+// each step is recomputed and then reversed stage by stage.  The step-size
+// controller's decisions carry no derivative; every accepted step keeps its
+// fraction of [t0, t1] while t0 (start time), t1 (last output time), the
+// initial state, the cosmology scalars and every spline knot move.
+//
+// Parameter derivatives are projected on up to MAXDIR input tangent
+// directions (d cosmo, d tables, d tau_out per direction, supplied by the
+// caller): every contraction below runs in dn<1+MAXDIR> arithmetic whose
+// component 0 is the derivative along the scale factor a (or tau) and whose
+// components 1..MAXDIR are the input tangents.
+#pragma once
+
+#include "deb_forward.h"
+
+namespace deb {
+
+constexpr int MAXDIR = DEB_MAXDIR;
+using TD = dn<1 + MAXDIR>;          // (a or tau, theta_1..theta_MAXDIR)
+using TH = dn<1 + MAXDIR, dual>;    // the same, each component also differentiated along a
+
+// Rodas5 tableau in flattened form (constant memory, read with wave-uniform
+// indices): stage point u_i = y0 + sum_j kA[i][j] k_j, y1 = y0 + sum_j kA8[j] k_j,
+// stage time t0 + kc[i] h, kd[i] h dT added in stage i, kC[i][j] / h k_j.
+__constant__ double kA[8][8] = {
+    {0, 0, 0, 0, 0, 0, 0, 0},
+    {R5::a21, 0, 0, 0, 0, 0, 0, 0},
+    {R5::a31, R5::a32, 0, 0, 0, 0, 0, 0},
+    {R5::a41, R5::a42, R5::a43, 0, 0, 0, 0, 0},
+    {R5::a51, R5::a52, R5::a53, R5::a54, 0, 0, 0, 0},
+    {R5::a61, R5::a62, R5::a63, R5::a64, R5::a65, 0, 0, 0},
+    {R5::a61, R5::a62, R5::a63, R5::a64, R5::a65, 1.0, 0, 0},
+    {R5::a61, R5::a62, R5::a63, R5::a64, R5::a65, 1.0, 1.0, 0}};
+__constant__ double kA8[8] = {R5::a61, R5::a62, R5::a63, R5::a64, R5::a65, 1.0, 1.0, 1.0};
+__constant__ double kC[8][8] = {
+    {0, 0, 0, 0, 0, 0, 0, 0},
+    {R5::C21, 0, 0, 0, 0, 0, 0, 0},
+    {R5::C31, R5::C32, 0, 0, 0, 0, 0, 0},
+    {R5::C41, R5::C42, R5::C43, 0, 0, 0, 0, 0},
+    {R5::C51, R5::C52, R5::C53, R5::C54, 0, 0, 0, 0},
+    {R5::C61, R5::C62, R5::C63, R5::C64, R5::C65, 0, 0, 0},
+    {R5::C71, R5::C72, R5::C73, R5::C74, R5::C75, R5::C76, 0, 0},
+    {R5::C81, R5::C82, R5::C83, R5::C84, R5::C85, R5::C86, R5::C87, 0}};
+__constant__ double kc[8] = {0.0, R5::c2, R5::c3, R5::c4, R5::c5, 1.0, 1.0, 1.0};
+__constant__ double kd[8] = {R5::d1, R5::d2, R5::d3, R5::d4, R5::d5, 0.0, 0.0, 0.0};
+
+// ---------------------------------------------------------------------------
+// cross-lane moves and reductions of composite scalars, component by
+// component (member access only)
+template <class T>
+struct ncomp;
+template <>
+struct ncomp<double> { static constexpr int value = 1; };
+template <>
+struct ncomp<dual> { static constexpr int value = 2; };
+template <int N, class B>
+struct ncomp<dn<N, B>> { static constexpr int value = (N + 1) * ncomp<B>::value; };
+
+__device__ __forceinline__ double &comp(double &x, int) { return x; }
+__device__ __forceinline__ double &comp(dual &x, int i) { return i == 0 ? x.v : x.d; }
+template <int N, class B>
+__device__ __forceinline__ double &comp(dn<N, B> &x, int i) {
+  constexpr int cb = ncomp<B>::value;
+  return i < cb ? comp(x.v, i) : comp(x.d[(i - cb) / cb], (i - cb) % cb);
+}
+
+template <class T>
+__device__ __forceinline__ T t_from_prev(T x) {
+#pragma unroll
+  for (int i = 0; i < ncomp<T>::value; ++i) comp(x, i) = from_prev(comp(x, i));
+  return x;
+}
+template <class T>
+__device__ __forceinline__ T t_from_next(T x) {
+#pragma unroll
+  for (int i = 0; i < ncomp<T>::value; ++i) comp(x, i) = from_next(comp(x, i));
+  return x;
+}
+template <class T>
+__device__ __forceinline__ T t_readlane(T x, int lane) {
+#pragma unroll
+  for (int i = 0; i < ncomp<T>::value; ++i) comp(x, i) = readlane(comp(x, i), lane);
+  return x;
+}
+// wave-uniform sums of NV composite values
+template <class T, int NV>
+__device__ __forceinline__ void t_wave_sum(T (&v)[NV]) {
+  constexpr int C = ncomp<T>::value;
+  double flat[NV * C];
+#pragma unroll
+  for (int j = 0; j < NV; ++j)
+#pragma unroll
+    for (int i = 0; i < C; ++i) flat[j * C + i] = comp(v[j], i);
+  wave_sum<NV * C>(flat);
+#pragma unroll
+  for (int j = 0; j < NV; ++j)
+#pragma unroll
+    for (int i = 0; i < C; ++i) comp(v[j], i) = flat[j * C + i];
+}
+
+// ---------------------------------------------------------------------------
+// transposed structured solves: the forward solves' operations in reverse
+// order, each replaced by its transpose (same factors, no new factorisation)
+
+// x <- W_B^{-T} x
+template <int NS>
+__device__ __forceinline__ void bsolveT(const Mode<NS> &m, double (&x)[NS]) {
+  const int lane = lane_id();
+  // leaf back-substitution x_dg = (r_dg - wdg x_tg) ddg
+  const double xdg = readlane(x[0], L_DG);
+  double rdgb = m.ddg * xdg;
+  if (lane == L_TG) x[0] -= m.wdg * m.ddg * xdg;
+  if (lane == L_DG) x[0] = 0.0;
+#pragma unroll
+  for (int j = 0; j < NS; ++j) x[j] *= m.binv[j];
+  // PCR levels in reverse: x_i <- sg_i x_i + al_i x_{i-s} + be_i x_{i+s}
+#pragma unroll
+  for (int L = PCR_LEVELS - 1; L >= 0; --L) {
+    if (L >= m.nlev) continue;
+    const int s = 1 << L;
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+      const double v = m.al[L][j] * x[j];
+      const double w = m.be[L][j] * x[j];
+      double vn, wp;
+      if (L == 0) {
+        vn = from_next(v);
+        wp = from_prev(w);
+      } else {
+        const int lp = lane + s, lm = lane - s;
+        vn = from_lane(v, lp > WAVE - 1 ? WAVE - 1 : lp);
+        wp = from_lane(w, lm < 0 ? 0 : lm);
+        if (lp > WAVE - 1) vn = 0.0;
+        if (lm < 0) wp = 0.0;
+      }
+      const double sg = (L == 0) ? m.sig0[j] : 1.0;
+      x[j] = sg * x[j] + vn + wp;
+    }
+  }
+  // leaf pre-elimination x_tg -= leafm r_dg
+  rdgb -= m.leafm * readlane(x[0], L_TG);
+  if (lane == L_DG) x[0] += rdgb;
+}
+
+// x <- W^{-T} x (transpose of wsolve)
+template <int NS>
+__device__ __forceinline__ void wsolveT(const Mode<NS> &m, double (&x)[NS]) {
+  const int lane = lane_id();
+  const double kba = readlane(x[0], L_A), kb1 = readlane(x[0], L_AHP);
+  if (lane == L_A || lane == L_AHP) x[0] = 0.0;
+  const double r1b = m.gh * kb1;
+  double kab = kba + m.gh * m.J1a * kb1;
+  const double c1kb = m.gh * kb1;
+  double part[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    part[0] += m.Z[0][s] * x[s];
+    part[1] += m.Z[1][s] * x[s];
+    part[2] += m.Z[2][s] * x[s];
+  }
+  wave_sum<3>(part);
+  double tt[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) tt[r] = m.c1Z[r] * c1kb + part[r];
+  double gb[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) gb[c] = m.Sinv[0][c] * tt[0] + m.Sinv[1][c] * tt[1] + m.Sinv[2][c] * tt[2];
+#pragma unroll
+  for (int s = 0; s < NS; ++s)
+    x[s] += m.Cf[0][s] * gb[0] + m.Cf[1][s] * gb[1] + m.Cf[2][s] * gb[2] + m.c1[s] * c1kb;
+  bsolveT<NS>(m, x);
+  double pj = 0.0;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) pj += m.jcol[s] * x[s];
+  kab += wave_sum1(pj);
+  if (lane == L_A) x[0] = m.inv_kaden * kab;
+  if (lane == L_AHP) x[0] = r1b;
+}
+
+// ---------------------------------------------------------------------------
+// right-hand side contractions
+
+// sum over all rows of w_row f_row(a, Y): f is model_synchronous at scale
+// factor a (a T with the caller's seeds) for the state Y; exact in T to any
+// order (the functionals are carried in T).
+template <int NS, class T>
+__device__ __forceinline__ T contract(Ctx &cx, const uint32_t (&rd)[NS], T a, double invtau, const double (&Y)[NS],
+                                      const double (&w)[NS]) {
+  const int lane = lane_id();
+  Bg<T> b = background<T>(cx, a);
+  // [0] local stencil part, [1..3] sum w*u{h,eta,pi}, [4..7] sum c{h,eta,pi,1}*Y
+  T acc[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) acc[i] = tconst<T>(0.0);
+  const double ydg = readlane(Y[0], L_DG), ytg = readlane(Y[0], L_TG);
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    Coef<T> r = row_coef<T>(rd[s], b, cx, invtau);
+    const double ym = from_prev(Y[s]), yp = from_next(Y[s]);
+    T loc = r.sub * ym + r.diag * Y[s] + r.sup * yp;
+    if (s == 0) {
+      if (lane == L_DG) loc = loc + r.leaf * ytg;
+      if (lane == L_TG) loc = loc + r.leaf * ydg;
+      if (lane == L_A) loc = b.aH * b.a;
+    }
+    acc[0] = acc[0] + loc * w[s];
+    acc[1] = acc[1] + r.uh * w[s];
+    acc[2] = acc[2] + r.ueta * w[s];
+    acc[3] = acc[3] + r.upi * w[s];
+    acc[4] = acc[4] + r.ch * Y[s];
+    acc[5] = acc[5] + r.ceta * Y[s];
+    acc[6] = acc[6] + r.cpi * Y[s];
+    acc[7] = acc[7] + r.c1 * Y[s];
+  }
+  t_wave_sum<T, 8>(acc);
+  const double wahp = readlane(w[0], L_AHP);
+  return acc[0] + acc[1] * acc[4] + acc[2] * acc[5] + acc[3] * acc[6] + acc[7] * wahp;
+}
+
+// out = A(a)^T w for the y-linear part of f (rows other than a; the a-column
+// and the a-row are handled by contract).  T = dual gives (A^T w, A'(a)^T w).
+template <int NS, class T>
+__device__ __forceinline__ void tapply(Ctx &cx, const uint32_t (&rd)[NS], T a, double invtau, const double (&w)[NS],
+                                       T (&out)[NS]) {
+  const int lane = lane_id();
+  Bg<T> b = background<T>(cx, a);
+  T su[3];
+  su[0] = tconst<T>(0.0);
+  su[1] = tconst<T>(0.0);
+  su[2] = tconst<T>(0.0);
+  T cc[4][NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    Coef<T> r = row_coef<T>(rd[s], b, cx, invtau);
+    T sw = r.sub * w[s];
+    T pw = r.sup * w[s];
+    out[s] = r.diag * w[s] + t_from_next(sw) + t_from_prev(pw);
+    if (s == 0) {
+      T lw = r.leaf * w[0];
+      T l_dg = t_readlane(lw, L_DG), l_tg = t_readlane(lw, L_TG);
+      if (lane == L_TG) out[0] = out[0] + l_dg;
+      if (lane == L_DG) out[0] = out[0] + l_tg;
+    }
+    su[0] = su[0] + r.uh * w[s];
+    su[1] = su[1] + r.ueta * w[s];
+    su[2] = su[2] + r.upi * w[s];
+    cc[0][s] = r.ch;
+    cc[1][s] = r.ceta;
+    cc[2][s] = r.cpi;
+    cc[3][s] = r.c1;
+  }
+  t_wave_sum<T, 3>(su);
+  const double wahp = readlane(w[0], L_AHP);
+#pragma unroll
+  for (int s = 0; s < NS; ++s) out[s] = out[s] + cc[0][s] * su[0] + cc[1][s] * su[1] + cc[2][s] * su[2] + cc[3][s] * wahp;
+}
+
+// per-lane multiplier of the explicit tau dependence (hierarchy truncations):
+// d f_row / d tau = trunc_row / tau^2 * y_row with trunc_row = l + 1
+template <int NS>
+__device__ __forceinline__ void trunc_weights(const uint32_t (&rd)[NS], double (&tw)[NS]) {
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    const uint32_t t = rd_type(rd[s]);
+    const bool trunc = (t == R_FLMAX) | (t == R_GLMAX) | (t == R_FRLMAX) | (t == R_QLMAX);
+    tw[s] = trunc ? (rd_l(rd[s]) + 1.0) : 0.0;
+  }
+}
+
+__device__ __forceinline__ TD seed_td(double v, double d0) {
+  TD r;
+  r = v;
+  r.d[0] = d0;
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// reverse of one Rodas5 step.  In: ybar = dL/dy1.  Out: ybar = dL/dy0,
+// tbar += dL/dt0 (h fixed), hbar += dL/dh, thb[j] += dL/dtheta_j.
+// Kl / KBl: per-wave LDS scratch of 8 and 7 stage vectors.
+template <int NS>
+__device__ __forceinline__ void rodas5_step_vjp(Mode<NS> &m, Ctx &cx, double t0, double h, const double (&y0)[NS],
+                                                double (&ybar)[NS], double (&thb)[MAXDIR], double &tbar, double &hbar,
+                                                double *Kl, double *KBl) {
+  const int lane = lane_id();
+  auto Kp = [&](int i, int s) -> double & { return Kl[(i * NS + s) * WAVE + lane]; };
+  auto KBp = [&](int i, int s) -> double & { return KBl[(i * NS + s) * WAVE + lane]; };
+  // forward recompute, keeping every stage vector
+  {
+    double y1[NS], err[NS];
+    rodas5_step_s<NS>(m, cx, t0, h, y0, y1, err, [&](int i, const double (&k)[NS]) {
+#pragma unroll
+      for (int s = 0; s < NS; ++s) Kp(i, s) = k[s];
+    });
+  }
+  double tw[NS];
+  trunc_weights<NS>(m.rd, tw);
+  const double it0 = 1.0 / t0;
+  const double a0 = readlane(y0[0], L_A);
+  double dT[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) dT[s] = tw[s] * it0 * it0 * y0[s];
+  const double gamma = R5::gamma;
+  const double ih = 1.0 / h;
+  double y1b[NS], y0b[NS], dTb[NS], rho[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    y1b[s] = ybar[s];
+    y0b[s] = ybar[s];  // y1 = y0 + sum A8 k
+    dTb[s] = 0.0;
+    rho[s] = 0.0;
+  }
+#pragma unroll
+  for (int j = 0; j < 7; ++j)
+#pragma unroll
+    for (int s = 0; s < NS; ++s) KBp(j, s) = kA8[j] * y1b[s];
+  double th_acc[MAXDIR];
+#pragma unroll
+  for (int j = 0; j < MAXDIR; ++j) th_acc[j] = 0.0;
+  double a0b = 0.0, t0b = 0.0, hb = 0.0;
+#pragma nounroll
+  for (int i = 7; i >= 0; --i) {
+    double rb[NS], ki[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      rb[s] = (i == 7) ? y1b[s] * kA8[7] : KBp(i, s);
+      ki[s] = Kp(i, s);
+    }
+    wsolveT<NS>(m, rb);
+    // stage point u_i and the combination sum_j C_ij k_j
+    double u[NS], ck[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      u[s] = y0[s];
+      ck[s] = 0.0;
+    }
+    for (int j = 0; j < i; ++j) {
+      const double aij = kA[i][j], cij = kC[i][j];
+#pragma unroll
+      for (int s = 0; s < NS; ++s) {
+        const double kj = Kp(j, s);
+        u[s] += aij * kj;
+        ck[s] += cij * kj;
+        KBp(j, s) += (cij * ih) * rb[s];
+      }
+    }
+    const double ti = t0 + kc[i] * h;
+    const double iti = 1.0 / ti;
+    // dot products: h-derivative of W, C/h and d h dT terms; explicit time
+    // dependence at the stage point and of J at (t0, y0)
+    double dots[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      dots[0] += rb[s] * (ki[s] / (gamma * h * h) - ck[s] * ih * ih + kd[i] * dT[s]);
+      dots[1] += rb[s] * tw[s] * iti * iti * u[s];
+      dots[2] += rb[s] * tw[s] * it0 * it0 * ki[s];
+    }
+    wave_sum<3>(dots);
+    hb += dots[0] + kc[i] * dots[1];
+    t0b += dots[1] + dots[2];
+    if (kd[i] != 0.0) {
+#pragma unroll
+      for (int s = 0; s < NS; ++s) dTb[s] += kd[i] * h * rb[s];
+    }
+    // f(t_i, u_i): ubar = J(t_i, u_i)^T rbar
+    double ub[NS];
+    {
+      const double ai = readlane(u[0], L_A);
+      tapply<NS, double>(cx, m.rd, ai, iti, rb, ub);
+      TD g = contract<NS, TD>(cx, m.rd, seed_td(ai, 1.0), iti, u, rb);
+      if (lane == L_A) ub[0] += g.d[0];
+#pragma unroll
+      for (int j = 0; j < MAXDIR; ++j) th_acc[j] += g.d[j + 1];
+    }
+#pragma unroll
+    for (int s = 0; s < NS; ++s) y0b[s] += ub[s];
+    for (int j = 0; j < i; ++j) {
+      const double aij = kA[i][j];
+#pragma unroll
+      for (int s = 0; s < NS; ++s) KBp(j, s) += aij * ub[s];
+    }
+    // J(t0, y0) k_i: the y-linear part at a0 (the a-column goes through rho)
+    {
+      double wr[NS];
+#pragma unroll
+      for (int s = 0; s < NS; ++s) wr[s] = rb[s];
+      if (lane == L_A) wr[0] = 0.0;
+      TD g = contract<NS, TD>(cx, m.rd, seed_td(a0, 1.0), it0, ki, wr);
+      a0b += g.d[0];
+#pragma unroll
+      for (int j = 0; j < MAXDIR; ++j) th_acc[j] += g.d[j + 1];
+      const double kia = readlane(ki[0], L_A);
+#pragma unroll
+      for (int s = 0; s < NS; ++s) rho[s] += kia * rb[s];
+    }
+  }
+  // a-column of J(t0, y0): rho^T df/da, differentiated along y0, a0 and theta
+  {
+    dual ad;
+    ad.v = a0;
+    ad.d = 1.0;
+    dual gy[NS];
+    tapply<NS, dual>(cx, m.rd, ad, it0, rho, gy);
+#pragma unroll
+    for (int s = 0; s < NS; ++s) y0b[s] += gy[s].d;
+    TH ah;
+    ah = a0;
+    ah.v.d = 1.0;
+    ah.d[0].v = 1.0;
+    TH g = contract<NS, TH>(cx, m.rd, ah, it0, y0, rho);
+    a0b += g.d[0].d;
+#pragma unroll
+    for (int j = 0; j < MAXDIR; ++j) th_acc[j] += g.d[j + 1].d;
+  }
+  // dT = D(t0) y0
+  {
+    double pt = 0.0;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      y0b[s] += tw[s] * it0 * it0 * dTb[s];
+      pt += dTb[s] * (-2.0 * tw[s] * it0 * it0 * it0) * y0[s];
+    }
+    t0b += wave_sum1(pt);
+  }
+  if (lane == L_A) y0b[0] += a0b;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) ybar[s] = y0b[s];
+#pragma unroll
+  for (int j = 0; j < MAXDIR; ++j) thb[j] += th_acc[j];
+  tbar += t0b;
+  hbar += hb;
+}
+
+// ---------------------------------------------------------------------------
+// start time and initial conditions in tangent arithmetic (the same formulas
+// as start_time and initial_conditions)
+
+template <class T>
+__device__ __forceinline__ T start_time_t(Ctx &cx, T tau_out_min) {
+  using P = ParT<T>;
+  const P taumin = cv<T>(cx, F_TAUMIN), YHe = cv<T>(cx, F_YHE), Omegab = cv<T>(cx, F_OMEGAB), H0 = cv<T>(cx, F_H0);
+  const T tau0 = taumin;
+  const T tau1 = spl_at<T>(cx, DEB_SPL_TAU_A, tconst<T>(0.1));
+  const double tau_k = 1.0 / cx.k;
+  const P akthom = 2.3048e-9 * (1.0 - YHe) * Omegab * H0 * H0;
+  T lt[2];
+#pragma unroll
+  for (int which = 0; which < 2; ++which) {
+    // the bisection compares signs only: the root's tangent flows through
+    // the bracket end points
+    T xl = dlog(tau0), xr = dlog(tau1);
+    auto fn = [&](const T &x) {
+      T tau = dexp(x);
+      T a = spl_at<T>(cx, DEB_SPL_A_TAU, tau);
+      Bg<T> b = background<T>(cx, a);
+      if (which == 0) return vval((1.0 / b.aH) / tau_k / 0.07 - 1.0);
+      T xe = spl_at<T>(cx, DEB_SPL_XE_TAU, tau);
+      T opac = xe * akthom / (a * a);
+      return vval((1.0 / opac) / (1.0 / b.aH) / 0.0004 - 1.0);
+    };
+    for (int i = 0; i < 7; ++i) {
+      T xm = 0.5 * (xl + xr);
+      if (fn(xm) * fn(xl) > 0) xl = xm; else xr = xm;
+    }
+    lt[which] = 0.5 * (xl + xr);
+  }
+  T ts = dexp(vval(lt[1]) < vval(lt[0]) ? lt[1] : lt[0]);
+  return 0.99 * (vval(tau_out_min) < vval(ts) ? tau_out_min : ts);
+}
+
+template <int NS, class T>
+__device__ __forceinline__ void ics_t(Ctx &cx, const uint32_t (&rd)[NS], T tau, T (&y)[NS]) {
+  using P = ParT<T>;
+  const P grhom = cv<T>(cx, F_GRHOM), grhog = cv<T>(cx, F_GRHOG), grhor = cv<T>(cx, F_GRHOR);
+  const P Omegam = cv<T>(cx, F_OMEGAM), Omegab = cv<T>(cx, F_OMEGAB), Neff = cv<T>(cx, F_NEFF);
+  const P Nmnu = cv<T>(cx, F_NMNU), amnu = cv<T>(cx, F_AMNU), w0 = cv<T>(cx, F_W0), wa = cv<T>(cx, F_WA);
+  const P cs2Q = cv<T>(cx, F_CS2DE);
+  const double k = cx.k;
+  T a = spl_at<T>(cx, DEB_SPL_A_TAU, tau);
+  T rnu = dexp(spl_at<T>(cx, DEB_SPL_LOGRHONU_LOGA, dlog(a)));
+  T a4 = a * a * a * a;
+  T rhom = grhom * Omegam / (a * a * a);
+  T rhor = (grhog + grhor * (Neff + Nmnu * rnu)) / a4;
+  T rhonu = grhor * (Neff + Nmnu * rnu) / a4;
+  P fracb = Omegab / Omegam;
+  T fracnu = rhonu / rhor;
+  T om = a * rhom / dsqrt(rhor);
+  const double C = -1.0;
+  T kt = k * tau;
+  T deltag = -(kt * kt) / 3.0 * (1.0 - om * tau / 5.0) * C;
+  T thetag = -(kt * kt * kt) / tau / 36.0 *
+             (1.0 - 3.0 * (1.0 + 5.0 * fracb - fracnu) / 20.0 / (1.0 - fracnu) * om * tau) * C;
+  T thetar = -(kt * kt * kt * kt) / tau / 36.0 / (4.0 * fracnu + 15.0) *
+             (4.0 * fracnu + 11.0 + 12.0 -
+              3.0 * (8.0 * fracnu * fracnu + 50.0 * fracnu + 275.0) / 20.0 / (2.0 * fracnu + 15.0) * tau * om) * C;
+  T shearr = (kt * kt) / (45.0 + 12.0 * fracnu) * 2.0 *
+             (1.0 + (4.0 * fracnu - 5.0) / 4.0 / (2.0 * fracnu + 15.0) * tau * om) * C;
+  T wQ = w0 + wa * (1.0 - a);
+  T deltaq = (kt * kt) / 4.0 * (1.0 + wQ) * (4.0 - 3.0 * cs2Q) / (4.0 - 6.0 * wQ + 3.0 * cs2Q) * C;
+  T thetaq = (kt * kt * kt * kt) / tau / 4.0 * cs2Q / (4.0 - 6.0 * wQ + 3.0 * cs2Q) * C;
+  if (cx.lambda) {
+    deltaq = tconst<T>(0.0);
+    thetaq = tconst<T>(0.0);
+  }
+  T eta = C * (1.0 - (kt * kt) / 12.0 / (15.0 + 4.0 * fracnu) *
+                         (5.0 + 4.0 * fracnu - (16.0 * fracnu * fracnu + 280.0 * fracnu + 325.0) / 10.0 /
+                                                   (2.0 * fracnu + 15.0) * tau * om));
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    const uint32_t t = rd_type(rd[s]);
+    const int qi = (int)rd_q(rd[s]);
+    T v = tconst<T>(0.0);
+    switch (t) {
+      case R_A: v = a; break;
+      case R_ETA: v = eta; break;
+      case R_DC: v = 0.75 * deltag; break;
+      case R_DB: v = 0.75 * deltag; break;
+      case R_TB: v = thetag; break;
+      case R_DG: v = deltag; break;
+      case R_TG: v = thetag; break;
+      case R_DR: v = deltag; break;
+      case R_TR: v = thetar; break;
+      case R_FR2: v = 2.0 * shearr; break;
+      case R_DQ: v = deltaq; break;
+      case R_TQ: v = thetaq; break;
+      case R_Q0: case R_Q1: case R_Q2: {
+        T aq = a * amnu / cx.q[qi];
+        T vq = 1.0 / dsqrt(1.0 + aq * aq);
+        const double dl = cx.dlnf[qi];
+        if (t == R_Q0) v = -0.25 * dl * deltag;
+        else if (t == R_Q1) v = -dl * thetar / vq / k / 3.0;
+        else v = -0.5 * dl * shearr;
+        break;
+      }
+      default: break;
+    }
+    y[s] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// the gradient kernel's arguments
+struct AdjArgs {
+  int ndir;
+  const double *dcos;     // [n_cosmo][ndir][24]
+  const double *dtab;     // [n_cosmo][ndir][stride]
+  const double *dtau;     // [n_cosmo][ndir][nout]
+  const double *ybar;     // [n_cosmo][nk][nout][20]  dL/d(output fields)
+  const int32_t *status;  // [n_cosmo][nk] of the taped forward solve
+  double *grad;           // [n_cosmo][nk][ndir]
+  TapeView tape;
+  int n_in;               // state entries read by the output conversion
+  int32_t in_idx[WAVE];   // their reference indices, one per lane
+};
+
+__device__ __forceinline__ void lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// reverse of the output conversion of one output time: yb = dL/dy (slot
+// layout), thb += dL/dtheta at fixed y.  w: the 20 field cotangents.
+template <int NS>
+__device__ __forceinline__ void output_vjp(const Params<NS> &P, Ctx &cx, const AdjArgs &A, const uint32_t (&rd)[NS],
+                                           const double (&yo)[NS], const double *w, double *ls, double *lg,
+                                           double (&yb)[NS], double (&thb)[MAXDIR]) {
+  const int lane = lane_id();
+  const int L0 = P.cfg.lmaxg, L1 = P.cfg.lmaxgp, L2 = P.cfg.lmaxr;
+  for (int i = lane; i < P.n; i += WAVE) lg[i] = 0.0;
+#pragma unroll
+  for (int s = 0; s < NS; ++s)
+    if (rd_type(rd[s]) != R_NONE) ls[rd_idx(rd[s])] = yo[s];
+  lds_sync();
+  double om[DEB_NOUT_FIELDS];
+#pragma unroll
+  for (int f = 0; f < DEB_NOUT_FIELDS; ++f) om[f] = w[f];
+  {
+    TD acc = tconst<TD>(0.0);
+    output_fields_t<TD>(cx, [&](int i) { return tconst<TD>(ls[i]); }, L0, L1, L2, P.nq, P.n,
+                        [&](int f, const TD &v) { acc = acc + v * om[f]; });
+#pragma unroll
+    for (int j = 0; j < MAXDIR; ++j) thb[j] += acc.d[j + 1];
+  }
+  if (lane < A.n_in) {
+    const int seed = A.in_idx[lane];
+    dual acc;
+    acc = 0.0;
+    output_fields_t<dual>(cx,
+                          [&](int i) {
+                            dual r;
+                            r.v = ls[i];
+                            r.d = (i == seed) ? 1.0 : 0.0;
+                            return r;
+                          },
+                          L0, L1, L2, P.nq, P.n, [&](int f, const dual &v) { acc = acc + v * om[f]; });
+    lg[seed] = acc.d;
+  }
+  lds_sync();
+#pragma unroll
+  for (int s = 0; s < NS; ++s) yb[s] = rd_type(rd[s]) != R_NONE ? lg[rd_idx(rd[s])] : 0.0;
+  lds_sync();
+}
+
+// one (cosmology, k) item of the gradient kernel
+template <int NS>
+__device__ __forceinline__ void adjoint_mode(const Params<NS> &P, const AdjArgs &A, int ci, int ki, double *Kl,
+                                             double *KBl, double *ls, double *lg) {
+  const int lane = lane_id();
+  const long long item = (long long)ci * P.nk + ki;
+  double *gout = A.grad + item * A.ndir;
+  if (A.status[item] != DEB_STATUS_OK || A.tape.count[item] < 2) {
+    if (lane < A.ndir) gout[lane] = __builtin_nan("");
+    return;
+  }
+  Ctx cx;
+  const double k = P.kmodes[item];
+  make_ctx<NS>(cx, P, ci, k);
+  cx.ndir = A.ndir;
+  cx.dcos = A.dcos + (long long)ci * A.ndir * COSMO_DOUBLES;
+  cx.dtab = A.dtab + (long long)ci * A.ndir * P.table_stride;
+  Mode<NS> m;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) m.rd[s] = P.rowdesc[s][lane];
+  m.nlev = mode_levels(P.cfg, P.nq);
+  const double *tout = P.tau_out + (long long)ci * P.nout;
+  const double *dtout = A.dtau + (long long)ci * A.ndir * P.nout;
+  int imin = 0, imax = 0;
+  for (int i = 1; i < P.nout; ++i) {
+    if (tout[i] < tout[imin]) imin = i;
+    if (tout[i] > tout[imax]) imax = i;
+  }
+  // tape reader: states r = N .. 0 of this mode, last chunk first
+  const TapeView &tv = A.tape;
+  const int N = tv.count[item] - 1;
+  long long chunk = tv.last[item];
+  int chunk_ord = N / TAPE_CH;
+  auto load = [&](int r, double &t, double (&y)[NS]) {
+    while (r / TAPE_CH < chunk_ord) {
+      chunk = *reinterpret_cast<const long long *>(tv.pool + chunk * tv.chunk_bytes);
+      --chunk_ord;
+    }
+    const double *tt = reinterpret_cast<const double *>(tv.pool + chunk * tv.chunk_bytes + 64);
+    const double *yy = tt + TAPE_CH;
+    const int slot = r % TAPE_CH;
+    t = tt[slot];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) y[s] = yy[(slot * NS + s) * WAVE + lane];
+  };
+  double ynext[NS], ycur[NS], yb[NS], pend[NS];
+  double tnext, tcur, t0 = 0.0;
+  load(N, tnext, ynext);
+  {
+    double tz;
+    double yz[NS];
+    // t0 is the first state's time
+    long long save_chunk = chunk;
+    int save_ord = chunk_ord;
+    load(0, tz, yz);
+    t0 = tz;
+    chunk = save_chunk;
+    chunk_ord = save_ord;
+  }
+  const double T1 = tout[imax];
+  const double span = T1 - t0;
+  double thb[MAXDIR];
+#pragma unroll
+  for (int j = 0; j < MAXDIR; ++j) thb[j] = 0.0;
+  double Tb0 = 0.0, Tb1 = 0.0;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) yb[s] = 0.0;
+  const double *wbase = A.ybar + item * (long long)P.nout * DEB_NOUT_FIELDS;
+  int iout = P.nout - 1;
+  for (int n = N - 1; n >= 0; --n) {
+    load(n, tcur, ycur);
+    const double h = tnext - tcur;
+    double tb = 0.0, hb = 0.0;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) pend[s] = 0.0;
+    // outputs interpolated inside this step (t_n < tau_o <= t_{n+1}; step 0 also
+    // takes tau_o <= t_1)
+    while (iout >= 0 && (tout[iout] > tcur || n == 0)) {
+      const double th = (tout[iout] - tcur) / h;
+      double yo[NS], ybo[NS];
+#pragma unroll
+      for (int s = 0; s < NS; ++s) yo[s] = ycur[s] + th * (ynext[s] - ycur[s]);
+      output_vjp<NS>(P, cx, A, m.rd, yo, wbase + (long long)iout * DEB_NOUT_FIELDS, ls, lg, ybo, thb);
+      double pth = 0.0;
+#pragma unroll
+      for (int s = 0; s < NS; ++s) {
+        yb[s] += th * ybo[s];
+        pend[s] += (1.0 - th) * ybo[s];
+        pth += ybo[s] * (ynext[s] - ycur[s]);
+      }
+      const double thbar = wave_sum1(pth);
+      const double taub = thbar / h;
+#pragma unroll
+      for (int j = 0; j < MAXDIR; ++j)
+        if (j < A.ndir) thb[j] += taub * dtout[j * P.nout + iout];
+      tb += -thbar / h;
+      hb += -thbar * th / h;
+      --iout;
+    }
+    rodas5_step_vjp<NS>(m, cx, tcur, h, ycur, yb, thb, tb, hb, Kl, KBl);
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      yb[s] += pend[s];
+      ynext[s] = ycur[s];
+    }
+    const double sn = (tcur - t0) / span, dsn = h / span;
+    Tb0 += tb * (1.0 - sn) - hb * dsn;
+    Tb1 += tb * sn + hb * dsn;
+    tnext = tcur;
+  }
+  // initial conditions y(t0) = IC(t0)
+  {
+    TD y0t[NS];
+    ics_t<NS, TD>(cx, m.rd, seed_td(t0, 1.0), y0t);
+    TD acc[1];
+    acc[0] = tconst<TD>(0.0);
+#pragma unroll
+    for (int s = 0; s < NS; ++s) acc[0] = acc[0] + y0t[s] * yb[s];
+    t_wave_sum<TD, 1>(acc);
+    Tb0 += acc[0].d[0];
+#pragma unroll
+    for (int j = 0; j < MAXDIR; ++j) thb[j] += acc[0].d[j + 1];
+  }
+  // t0 = start time(min tau_out), t1 = max tau_out
+  {
+    TD tmin;
+    tmin = tout[imin];
+#pragma unroll
+    for (int j = 0; j < MAXDIR; ++j) tmin.d[j + 1] = j < A.ndir ? dtout[j * P.nout + imin] : 0.0;
+    TD ts = start_time_t<TD>(cx, tmin);
+#pragma unroll
+    for (int j = 0; j < MAXDIR; ++j) {
+      thb[j] += Tb0 * ts.d[j + 1];
+      if (j < A.ndir) thb[j] += Tb1 * dtout[j * P.nout + imax];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < MAXDIR; ++j)
+    if (lane == j && j < A.ndir) gout[j] = thb[j];
+}
+
+}  // namespace deb

@@ -1,0 +1,313 @@
+// gfx950 kernels and C ABI of the gradient path.
+//
+//   deb_forward_tape_kernel  the forward solve of deb_forward_kernel that also
+//                            writes every accepted state to the checkpoint tape
+//   deb_adjoint_kernel       one wavefront per (cosmology, k) item: reads the
+//                            mode's states back last-to-first, reverses every
+//                            step, the output interpolation, the initial
+//                            conditions and the start time, and returns the
+//                            derivative of the caller's scalar along each
+//                            input tangent direction
+//   deb_rhs_vjp_kernel,
+//   deb_step_vjp_kernel      test hooks: reverse of one right-hand side
+//                            evaluation and of one Rodas5 step
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "deb_adjoint.h"
+#include "deb_host.h"
+
+namespace deb {
+
+template <int NS>
+__global__ void __launch_bounds__(WAVE) deb_forward_tape_kernel(const Params<NS> P, const TapeView T) {
+  __shared__ double lds[WAVE * NS];
+  const int item = blockIdx.x;
+  if (item >= P.n_cosmo * P.nk) return;
+  const int ki = P.nk - 1 - item / P.n_cosmo;
+  const int ci = item % P.n_cosmo;
+  integrate_mode<NS, true>(P, ci, ki, lds, &T);
+}
+
+template <int NS>
+__global__ void __launch_bounds__(WAVE) deb_adjoint_kernel(const Params<NS> P, const AdjArgs A) {
+  __shared__ double Kl[8 * NS * WAVE];
+  __shared__ double KBl[7 * NS * WAVE];
+  __shared__ double ls[NS * WAVE];
+  __shared__ double lg[NS * WAVE];
+  const int item = blockIdx.x;
+  if (item >= P.n_cosmo * P.nk) return;
+  const int ki = P.nk - 1 - item / P.n_cosmo;
+  const int ci = item % P.n_cosmo;
+  adjoint_mode<NS>(P, A, ci, ki, Kl, KBl, ls, lg);
+}
+
+struct VjpHook {
+  const int32_t *ci;
+  const double *kk, *t0, *t1, *y, *wbar;
+  double *ybar, *t0bar, *t1bar, *grad;
+  int m, ndir;
+  const double *dcos, *dtab;
+};
+
+template <int NS>
+__device__ __forceinline__ void hook_ctx(Ctx &cx, const Params<NS> &P, const VjpHook &H, int item) {
+  const int ci = H.ci[item];
+  make_ctx<NS>(cx, P, ci, H.kk[item]);
+  cx.ndir = H.ndir;
+  cx.dcos = H.dcos + (long long)ci * H.ndir * COSMO_DOUBLES;
+  cx.dtab = H.dtab + (long long)ci * H.ndir * P.table_stride;
+}
+
+template <int NS>
+__global__ void __launch_bounds__(WAVE) deb_rhs_vjp_kernel(const Params<NS> P, const VjpHook H) {
+  const int item = blockIdx.x;
+  if (item >= H.m) return;
+  const int lane = lane_id();
+  Ctx cx;
+  hook_ctx<NS>(cx, P, H, item);
+  uint32_t rd[NS];
+  double y[NS], w[NS], yb[NS], tw[NS];
+  const double *ys = H.y + (long long)item * P.n, *ws = H.wbar + (long long)item * P.n;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    rd[s] = P.rowdesc[s][lane];
+    const bool on = rd_type(rd[s]) != R_NONE;
+    y[s] = on ? ys[rd_idx(rd[s])] : 0.0;
+    w[s] = on ? ws[rd_idx(rd[s])] : 0.0;
+  }
+  const double tau = H.t0[item], it = 1.0 / tau;
+  const double a = readlane(y[0], L_A);
+  tapply<NS, double>(cx, rd, a, it, w, yb);
+  TD g = contract<NS, TD>(cx, rd, seed_td(a, 1.0), it, y, w);
+  if (lane == L_A) yb[0] += g.d[0];
+  trunc_weights<NS>(rd, tw);
+  double pt = 0.0;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) pt += w[s] * tw[s] * it * it * y[s];
+  pt = wave_sum1(pt);
+  double *yo = H.ybar + (long long)item * P.n;
+#pragma unroll
+  for (int s = 0; s < NS; ++s)
+    if (rd_type(rd[s]) != R_NONE) yo[rd_idx(rd[s])] = yb[s];
+  if (lane == 0) H.t0bar[item] = pt;
+#pragma unroll
+  for (int j = 0; j < MAXDIR; ++j)
+    if (lane == j && j < H.ndir) H.grad[(long long)item * H.ndir + j] = g.d[j + 1];
+}
+
+template <int NS>
+__global__ void __launch_bounds__(WAVE) deb_step_vjp_kernel(const Params<NS> P, const VjpHook H) {
+  __shared__ double Kl[8 * NS * WAVE];
+  __shared__ double KBl[7 * NS * WAVE];
+  const int item = blockIdx.x;
+  if (item >= H.m) return;
+  const int lane = lane_id();
+  Ctx cx;
+  hook_ctx<NS>(cx, P, H, item);
+  Mode<NS> m;
+  m.nlev = mode_levels(P.cfg, P.nq);
+  double y[NS], yb[NS];
+  const double *ys = H.y + (long long)item * P.n, *ws = H.wbar + (long long)item * P.n;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    m.rd[s] = P.rowdesc[s][lane];
+    const bool on = rd_type(m.rd[s]) != R_NONE;
+    y[s] = on ? ys[rd_idx(m.rd[s])] : 0.0;
+    yb[s] = on ? ws[rd_idx(m.rd[s])] : 0.0;
+  }
+  const double t0 = H.t0[item], h = H.t1[item] - t0;
+  double thb[MAXDIR];
+#pragma unroll
+  for (int j = 0; j < MAXDIR; ++j) thb[j] = 0.0;
+  double tb = 0.0, hb = 0.0;
+  rodas5_step_vjp<NS>(m, cx, t0, h, y, yb, thb, tb, hb, Kl, KBl);
+  double *yo = H.ybar + (long long)item * P.n;
+#pragma unroll
+  for (int s = 0; s < NS; ++s)
+    if (rd_type(m.rd[s]) != R_NONE) yo[rd_idx(m.rd[s])] = yb[s];
+  if (lane == 0) {
+    H.t0bar[item] = tb - hb;
+    H.t1bar[item] = hb;
+  }
+#pragma unroll
+  for (int j = 0; j < MAXDIR; ++j)
+    if (lane == j && j < H.ndir) H.grad[(long long)item * H.ndir + j] = thb[j];
+}
+
+// ---------------------------------------------------------------------------
+// host side
+
+inline int tape_view(TapeView &tv, void *tape, size_t bytes, long long n_modes, int ns) {
+  const long long pool = tape_pool_off(n_modes);
+  const long long cb = tape_chunk_bytes(ns);
+  if (!tape || (long long)bytes < pool + cb) return fail(DEB_E_ARG, "tape smaller than deb_tape_bytes");
+  char *base = static_cast<char *>(tape);
+  tv.counter = reinterpret_cast<unsigned long long *>(base);
+  tv.capacity = ((long long)bytes - pool) / cb;
+  tv.last = reinterpret_cast<long long *>(base + tape_last_off());
+  tv.count = reinterpret_cast<int32_t *>(base + tape_count_off(n_modes));
+  tv.pool = base + pool;
+  tv.chunk_bytes = cb;
+  return 0;
+}
+
+// reference indices of the state entries read by the output conversion
+inline int output_inputs(const deb_config_t &c, int32_t *idx) {
+  const int lg = c.lmaxg, lgp = c.lmaxgp, lr = c.lmaxr, nq = c.nqmax;
+  const int n = 12 + lg + lgp + lr + nq * (c.lmaxnu + 1);
+  const int ir = 9 + lg + lgp, iq0 = 10 + lg + lgp + lr;
+  int m = 0;
+  for (int i : {0, 2, 3, 4, 5, 6, 7, 8}) idx[m++] = i;
+  idx[m++] = ir;
+  idx[m++] = ir + 1;
+  for (int i = 0; i < nq; ++i) idx[m++] = iq0 + i;
+  for (int i = 0; i < nq; ++i) idx[m++] = iq0 + nq + i;
+  idx[m++] = n - 2;
+  idx[m++] = n - 1;
+  for (int i = m; i < WAVE; ++i) idx[i] = -1;
+  return m;
+}
+
+template <int NS>
+int run_forward_tape(const deb_config_t *cfg, const Layout &L, int n_cosmo, const deb_cosmo_t *cosmo,
+                     const double *tables, int64_t stride, const int32_t *desc, const double *q, const double *w,
+                     int nk, const double *kmodes, int nout, const double *tau_out, double *y_out, int32_t *status,
+                     int32_t *nsteps, void *tape, size_t tape_bytes, hipStream_t st) {
+  Params<NS> P;
+  if (int r = fill_params<NS>(P, cfg, L, n_cosmo, cosmo, tables, stride, desc, q, w, nk, kmodes, nout, tau_out))
+    return r;
+  P.y_out = y_out;
+  P.status = status;
+  P.nsteps = nsteps;
+  TapeView tv;
+  if (int r = tape_view(tv, tape, tape_bytes, (long long)n_cosmo * nk, NS)) return r;
+  if (int r = hip_check(hipMemsetAsync(tape, 0, 256, st), "tape header reset")) return r;
+  hipLaunchKernelGGL(deb_forward_tape_kernel<NS>, dim3((unsigned)((int64_t)n_cosmo * nk)), dim3(WAVE), 0, st, P, tv);
+  return hip_check(hipGetLastError(), "taped forward kernel launch");
+}
+
+template <int NS>
+int run_adjoint(const deb_config_t *cfg, const Layout &L, int n_cosmo, const deb_cosmo_t *cosmo, const double *tables,
+                int64_t stride, const int32_t *desc, const double *q, const double *w, int nk, const double *kmodes,
+                int nout, const double *tau_out, int ndir, const double *dcos, const double *dtab,
+                const double *dtau, const double *ybar, const int32_t *status, double *grad, const void *tape,
+                size_t tape_bytes, hipStream_t st) {
+  Params<NS> P;
+  if (int r = fill_params<NS>(P, cfg, L, n_cosmo, cosmo, tables, stride, desc, q, w, nk, kmodes, nout, tau_out))
+    return r;
+  AdjArgs A;
+  std::memset(&A, 0, sizeof(A));
+  A.ndir = ndir;
+  A.dcos = dcos;
+  A.dtab = dtab;
+  A.dtau = dtau;
+  A.ybar = ybar;
+  A.status = status;
+  A.grad = grad;
+  if (int r = tape_view(A.tape, const_cast<void *>(tape), tape_bytes, (long long)n_cosmo * nk, NS)) return r;
+  A.n_in = output_inputs(*cfg, A.in_idx);
+  hipLaunchKernelGGL(deb_adjoint_kernel<NS>, dim3((unsigned)((int64_t)n_cosmo * nk)), dim3(WAVE), 0, st, P, A);
+  return hip_check(hipGetLastError(), "adjoint kernel launch");
+}
+
+template <int NS>
+int run_vjp_hook(bool step, const deb_config_t *cfg, const Layout &L, const deb_cosmo_t *cosmo, const double *tables,
+                 int64_t stride, const int32_t *desc, const double *q, const double *w, const VjpHook &H,
+                 hipStream_t st) {
+  Params<NS> P;
+  if (int r = fill_params<NS>(P, cfg, L, 1, cosmo, tables, stride, desc, q, w, 1, nullptr, 1, nullptr)) return r;
+  if (step) hipLaunchKernelGGL(deb_step_vjp_kernel<NS>, dim3(H.m), dim3(WAVE), 0, st, P, H);
+  else hipLaunchKernelGGL(deb_rhs_vjp_kernel<NS>, dim3(H.m), dim3(WAVE), 0, st, P, H);
+  return hip_check(hipGetLastError(), step ? "step vjp kernel launch" : "rhs vjp kernel launch");
+}
+
+inline int check_dirs(int ndir, const double *dcos, const double *dtab) {
+  if (ndir < 1 || ndir > DEB_MAXDIR) return fail(DEB_E_ARG, "ndir must be in [1, DEB_MAXDIR]");
+  if (!dcos || !dtab) return fail(DEB_E_ARG, "null tangent arrays");
+  return 0;
+}
+
+}  // namespace deb
+
+using namespace deb;
+
+extern "C" {
+
+int deb_tape_bytes(const deb_config_t *cfg, int n_cosmo, int nk, int64_t n_states, size_t *bytes) {
+  if (!cfg || !bytes || n_cosmo < 1 || nk < 1 || n_states < 0) return fail(DEB_E_ARG, "invalid argument");
+  Layout L;
+  if (int r = build_layout(*cfg, L)) return r;
+  const long long n_modes = (long long)n_cosmo * nk;
+  const long long chunks = n_states / TAPE_CH + n_modes + 1;
+  *bytes = (size_t)(tape_pool_off(n_modes) + chunks * tape_chunk_bytes(L.ns));
+  return 0;
+}
+
+int deb_solve_forward_tape(const deb_config_t *cfg, int n_cosmo, const deb_cosmo_t *cosmo, const double *tables,
+                           int64_t table_stride, const int32_t *desc, const double *q, const double *w, int nk,
+                           const double *kmodes, int nout, const double *tau_out, double *y_out, int32_t *status,
+                           int32_t *nsteps, void *tape, size_t tape_bytes, void *stream) {
+  if (int r = check_common(cfg, n_cosmo, cosmo, tables, desc, q, w)) return r;
+  if (nk < 1 || nout < 1 || !kmodes || !tau_out || !y_out || !status) return fail(DEB_E_ARG, "invalid mode arrays");
+  if ((int64_t)n_cosmo * nk > 0x7fffffff) return fail(DEB_E_ARG, "too many modes");
+  Layout L;
+  if (int r = build_layout(*cfg, L)) return r;
+  hipStream_t st = (hipStream_t)stream;
+  DEB_DISPATCH(L.ns, (run_forward_tape<NS>(cfg, L, n_cosmo, cosmo, tables, table_stride, desc, q, w, nk, kmodes, nout,
+                                           tau_out, y_out, status, nsteps, tape, tape_bytes, st)))
+}
+
+int deb_solve_adjoint(const deb_config_t *cfg, int n_cosmo, const deb_cosmo_t *cosmo, const double *tables,
+                      int64_t table_stride, const int32_t *desc, const double *q, const double *w, int nk,
+                      const double *kmodes, int nout, const double *tau_out, int ndir, const double *dcosmo,
+                      const double *dtables, const double *dtau_out, const double *ybar, const int32_t *status,
+                      double *grad, const void *tape, size_t tape_bytes, void *stream) {
+  if (int r = check_common(cfg, n_cosmo, cosmo, tables, desc, q, w)) return r;
+  if (int r = check_dirs(ndir, dcosmo, dtables)) return r;
+  if (nk < 1 || nout < 1 || !kmodes || !tau_out || !dtau_out || !ybar || !status || !grad)
+    return fail(DEB_E_ARG, "invalid mode arrays");
+  if (cfg->return_full) return fail(DEB_E_ARG, "the gradient path takes cotangents of the 20 output fields");
+  if ((int64_t)n_cosmo * nk > 0x7fffffff) return fail(DEB_E_ARG, "too many modes");
+  Layout L;
+  if (int r = build_layout(*cfg, L)) return r;
+  hipStream_t st = (hipStream_t)stream;
+  DEB_DISPATCH(L.ns, (run_adjoint<NS>(cfg, L, n_cosmo, cosmo, tables, table_stride, desc, q, w, nk, kmodes, nout,
+                                      tau_out, ndir, dcosmo, dtables, dtau_out, ybar, status, grad, tape, tape_bytes,
+                                      st)))
+}
+
+int deb_rhs_vjp(const deb_config_t *cfg, const deb_cosmo_t *cosmo, const double *tables, int64_t table_stride,
+                const int32_t *desc, const double *q, const double *w, int ndir, const double *dcosmo,
+                const double *dtables, int m, const int32_t *ci, const double *kk, const double *tau, const double *y,
+                const double *fbar, double *ybar, double *taubar, double *grad, void *stream) {
+  if (int r = check_common(cfg, 1, cosmo, tables, desc, q, w)) return r;
+  if (int r = check_dirs(ndir, dcosmo, dtables)) return r;
+  if (m < 1 || !ci || !kk || !tau || !y || !fbar || !ybar || !taubar || !grad)
+    return fail(DEB_E_ARG, "invalid hook arrays");
+  Layout L;
+  if (int r = build_layout(*cfg, L)) return r;
+  VjpHook H{ci, kk, tau, nullptr, y, fbar, ybar, taubar, nullptr, grad, m, ndir, dcosmo, dtables};
+  hipStream_t st = (hipStream_t)stream;
+  DEB_DISPATCH(L.ns, (run_vjp_hook<NS>(false, cfg, L, cosmo, tables, table_stride, desc, q, w, H, st)))
+}
+
+int deb_rodas5_step_vjp(const deb_config_t *cfg, const deb_cosmo_t *cosmo, const double *tables,
+                        int64_t table_stride, const int32_t *desc, const double *q, const double *w, int ndir,
+                        const double *dcosmo, const double *dtables, int m, const int32_t *ci, const double *kk,
+                        const double *t0, const double *t1, const double *y0, const double *y1bar, double *y0bar,
+                        double *t0bar, double *t1bar, double *grad, void *stream) {
+  if (int r = check_common(cfg, 1, cosmo, tables, desc, q, w)) return r;
+  if (int r = check_dirs(ndir, dcosmo, dtables)) return r;
+  if (m < 1 || !ci || !kk || !t0 || !t1 || !y0 || !y1bar || !y0bar || !t0bar || !t1bar || !grad)
+    return fail(DEB_E_ARG, "invalid hook arrays");
+  Layout L;
+  if (int r = build_layout(*cfg, L)) return r;
+  VjpHook H{ci, kk, t0, t1, y0, y1bar, y0bar, t0bar, t1bar, grad, m, ndir, dcosmo, dtables};
+  hipStream_t st = (hipStream_t)stream;
+  DEB_DISPATCH(L.ns, (run_vjp_hook<NS>(true, cfg, L, cosmo, tables, table_stride, desc, q, w, H, st)))
+}
+
+}  // extern "C"

@@ -121,6 +121,171 @@ __device__ __forceinline__ double dval(double) { return 0.0; }
 __device__ __forceinline__ double dval(dual a) { return a.d; }
 
 // ----------------------------------------------------------------------------
+// multi-tangent dual number: value v and N directional derivatives d[0..N-1].
+// B is double, or `dual` for a second-order (nested) derivative.
+template <int N, class B = double>
+struct dn {
+  static constexpr int size = N;
+  B v;
+  B d[N];
+  __device__ __forceinline__ dn &operator=(double x) {
+    v = x;
+#pragma unroll
+    for (int i = 0; i < N; ++i) d[i] = 0.0;
+    return *this;
+  }
+};
+template <int N, class B>
+__device__ __forceinline__ dn<N, B> mkdn(B v) {
+  dn<N, B> r;
+  r.v = v;
+#pragma unroll
+  for (int i = 0; i < N; ++i) r.d[i] = 0.0;
+  return r;
+}
+template <int N, class B>
+__device__ __forceinline__ dn<N, B> operator+(const dn<N, B> &a, const dn<N, B> &b) {
+  dn<N, B> r;
+  r.v = a.v + b.v;
+#pragma unroll
+  for (int i = 0; i < N; ++i) r.d[i] = a.d[i] + b.d[i];
+  return r;
+}
+template <int N, class B>
+__device__ __forceinline__ dn<N, B> operator-(const dn<N, B> &a, const dn<N, B> &b) {
+  dn<N, B> r;
+  r.v = a.v - b.v;
+#pragma unroll
+  for (int i = 0; i < N; ++i) r.d[i] = a.d[i] - b.d[i];
+  return r;
+}
+template <int N, class B>
+__device__ __forceinline__ dn<N, B> operator-(const dn<N, B> &a) {
+  dn<N, B> r;
+  r.v = -a.v;
+#pragma unroll
+  for (int i = 0; i < N; ++i) r.d[i] = -a.d[i];
+  return r;
+}
+template <int N, class B>
+__device__ __forceinline__ dn<N, B> operator*(const dn<N, B> &a, const dn<N, B> &b) {
+  dn<N, B> r;
+  r.v = a.v * b.v;
+#pragma unroll
+  for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * b.v + a.v * b.d[i];
+  return r;
+}
+template <int N, class B>
+__device__ __forceinline__ dn<N, B> operator/(const dn<N, B> &a, const dn<N, B> &b) {
+  dn<N, B> r;
+  B ib = 1.0 / b.v;
+  r.v = a.v * ib;
+#pragma unroll
+  for (int i = 0; i < N; ++i) r.d[i] = (a.d[i] - r.v * b.d[i]) * ib;
+  return r;
+}
+template <int N, class B>
+__device__ __forceinline__ dn<N, B> operator+(const dn<N, B> &a, double b) {
+  dn<N, B> r = a;
+  r.v = a.v + b;
+  return r;
+}
+template <int N, class B>
+__device__ __forceinline__ dn<N, B> operator+(double b, const dn<N, B> &a) { return a + b; }
+template <int N, class B>
+__device__ __forceinline__ dn<N, B> operator-(const dn<N, B> &a, double b) {
+  dn<N, B> r = a;
+  r.v = a.v - b;
+  return r;
+}
+template <int N, class B>
+__device__ __forceinline__ dn<N, B> operator-(double b, const dn<N, B> &a) {
+  dn<N, B> r;
+  r.v = b - a.v;
+#pragma unroll
+  for (int i = 0; i < N; ++i) r.d[i] = -a.d[i];
+  return r;
+}
+template <int N, class B>
+__device__ __forceinline__ dn<N, B> operator*(const dn<N, B> &a, double b) {
+  dn<N, B> r;
+  r.v = a.v * b;
+#pragma unroll
+  for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * b;
+  return r;
+}
+template <int N, class B>
+__device__ __forceinline__ dn<N, B> operator*(double b, const dn<N, B> &a) { return a * b; }
+template <int N, class B>
+__device__ __forceinline__ dn<N, B> operator/(const dn<N, B> &a, double b) { return a * (1.0 / b); }
+template <int N, class B>
+__device__ __forceinline__ dn<N, B> operator/(double b, const dn<N, B> &a) {
+  dn<N, B> r;
+  B ia = 1.0 / a.v;
+  r.v = b * ia;
+#pragma unroll
+  for (int i = 0; i < N; ++i) r.d[i] = -r.v * a.d[i] * ia;
+  return r;
+}
+template <int N, class B>
+__device__ __forceinline__ dn<N, B> dexp(const dn<N, B> &a) {
+  dn<N, B> r;
+  r.v = dexp(a.v);
+#pragma unroll
+  for (int i = 0; i < N; ++i) r.d[i] = r.v * a.d[i];
+  return r;
+}
+template <int N, class B>
+__device__ __forceinline__ dn<N, B> dlog(const dn<N, B> &a) {
+  dn<N, B> r;
+  r.v = dlog(a.v);
+  B ia = 1.0 / a.v;
+#pragma unroll
+  for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * ia;
+  return r;
+}
+template <int N, class B>
+__device__ __forceinline__ dn<N, B> dsqrt(const dn<N, B> &a) {
+  dn<N, B> r;
+  r.v = dsqrt(a.v);
+  B h = 0.5 / r.v;
+#pragma unroll
+  for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * h;
+  return r;
+}
+template <int N, class B>
+__device__ __forceinline__ dn<N, B> dpow(const dn<N, B> &a, double p) {
+  dn<N, B> r;
+  r.v = dpow(a.v, p);
+  B g = p * dpow(a.v, p - 1.0);
+#pragma unroll
+  for (int i = 0; i < N; ++i) r.d[i] = g * a.d[i];
+  return r;
+}
+template <int N, class B>
+__device__ __forceinline__ double vval(const dn<N, B> &a) { return vval(a.v); }
+
+template <class T>
+__device__ __forceinline__ T tconst(double v) {
+  T r;
+  r = v;
+  return r;
+}
+
+template <class T>
+struct is_dn { static constexpr bool value = false; };
+template <int N, class B>
+struct is_dn<dn<N, B>> { static constexpr bool value = true; };
+// scalar type of the cosmology parameters and spline knots when computing in
+// T: tangent-carrying for dn, plain double otherwise
+template <class T>
+struct par_of { using type = double; };
+template <int N, class B>
+struct par_of<dn<N, B>> { using type = dn<N, B>; };
+template <class T>
+using ParT = typename par_of<T>::type;
+
+// ----------------------------------------------------------------------------
 // cross-lane primitives (doubles moved as two 32-bit halves)
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
@@ -155,27 +320,61 @@ __device__ __forceinline__ double readfirst(double x) {
   return __hiloint2double(hi, lo);
 }
 
-// sum over the 16 lanes of each DPP row, result in every lane of the row
+// Wave-wide sums by a symmetric butterfly: xor 1 and xor 2 (quad_perm), the
+// half-row and row mirrors, then xor 16 and xor 32 (ds_bpermute).  Every
+// partial sum pairs two equal-sized groups, so each lane ends with the same
+// bits; the results stay in vector registers (no scalar-register pressure).
 __device__ __forceinline__ double row_sum16(double x) {
   x += dpp_f64<0xB1, true>(x);   // quad_perm [1,0,3,2]
   x += dpp_f64<0x4E, true>(x);   // quad_perm [2,3,0,1]
-  x += dpp_f64<0x124, true>(x);  // row_ror:4
-  x += dpp_f64<0x128, true>(x);  // row_ror:8
+  x += dpp_f64<0x141, true>(x);  // row_half_mirror
+  x += dpp_f64<0x140, true>(x);  // row_mirror
   return x;
 }
-// wave-uniform sum of N values
 template <int N>
 __device__ __forceinline__ void wave_sum(double (&v)[N]) {
+  const int lane = lane_id();
 #pragma unroll
   for (int i = 0; i < N; ++i) v[i] = row_sum16(v[i]);
 #pragma unroll
-  for (int i = 0; i < N; ++i)
-    v[i] = (readlane(v[i], 0) + readlane(v[i], 16)) + (readlane(v[i], 32) + readlane(v[i], 48));
+  for (int i = 0; i < N; ++i) v[i] += from_lane(v[i], lane ^ 16);
+#pragma unroll
+  for (int i = 0; i < N; ++i) v[i] += from_lane(v[i], lane ^ 32);
 }
+// one wave-wide sum, returned as a wave-uniform (scalar) value
 __device__ __forceinline__ double wave_sum1(double x) {
   double v[1] = {x};
   wave_sum<1>(v);
-  return v[0];
+  return readfirst(v[0]);
+}
+
+// ----------------------------------------------------------------------------
+// checkpoint tape of the gradient path: every accepted state of every mode.
+// Chunks of TAPE_CH states are handed out from one pool by an atomic counter
+// and linked backwards, so a mode's states are read back last-to-first.
+//   chunk: [int64 prev chunk (-1: none)][pad to 64 B][double t[CH]][double y[CH][NS][64]]
+constexpr int TAPE_CH = 16;
+
+__host__ __device__ inline long long tape_chunk_bytes(int ns) {
+  long long b = 64 + 8LL * TAPE_CH + 8LL * TAPE_CH * ns * WAVE;
+  return (b + 255) / 256 * 256;
+}
+
+struct TapeView {
+  unsigned long long *counter;  // chunks handed out so far
+  long long capacity;           // chunks in the pool
+  long long *last;              // [n_modes] last chunk of a mode
+  int32_t *count;               // [n_modes] states recorded (-1: pool exhausted)
+  char *pool;
+  long long chunk_bytes;
+};
+
+// byte layout of a tape buffer for n_modes modes: header, per-mode arrays, pool
+__host__ __device__ inline long long tape_last_off() { return 256; }
+__host__ __device__ inline long long tape_count_off(long long n_modes) { return 256 + 8 * n_modes; }
+__host__ __device__ inline long long tape_pool_off(long long n_modes) {
+  long long b = 256 + 12 * n_modes;
+  return (b + 255) / 256 * 256;
 }
 
 // ----------------------------------------------------------------------------
@@ -183,6 +382,7 @@ __device__ __forceinline__ double wave_sum1(double x) {
 struct Spl {
   const double *x, *y, *S;
   int n;
+  int off;  // offset of the knots in the cosmology's table block
 };
 
 // interval i with x[i] < v <= x[i+1], clipped to [0, n-2]

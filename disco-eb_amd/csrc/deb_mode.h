@@ -20,7 +20,73 @@ struct Ctx {
   const double *q, *w, *dlnf;
   int hint[DEB_NSPLINE];
   bool lambda;  // w0 == -1 and wa == 0: dark-energy perturbations frozen
+  // input tangents (gradient kernels): dcos[ndir][24], dtab[ndir][stride]
+  const double *dcos;
+  const double *dtab;
+  int64_t stride;
+  int ndir;
 };
+
+// deb_cosmo_t field indices
+enum : int {
+  F_GRHOM = 0, F_GRHOG, F_GRHOR, F_OMEGAM, F_OMEGAB, F_OMEGADE, F_OMEGAK, F_NEFF, F_NMNU, F_AMNU, F_W0, F_WA,
+  F_CS2DE, F_YHE, F_H0, F_TAUMIN
+};
+constexpr int COSMO_DOUBLES = 24;
+
+// cosmology scalar f as a T-compatible parameter: plain double, or (for dn
+// types) the value with its input tangents in components 1..ndir
+template <class T>
+__device__ __forceinline__ ParT<T> cv(const Ctx &cx, int f) {
+  const double v = reinterpret_cast<const double *>(cx.c)[f];
+  if constexpr (is_dn<T>::value) {
+    T r;
+    r = v;
+#pragma unroll
+    for (int j = 0; j + 1 < T::size; ++j)
+      if (j < cx.ndir) r.d[j + 1] = cx.dcos[j * COSMO_DOUBLES + f];
+    return r;
+  } else {
+    return v;
+  }
+}
+
+// spline s at x: dn types also carry the knots' input tangents (x, y and S)
+template <class T>
+__device__ __forceinline__ T spl_at(Ctx &cx, int s, T xv) {
+  if constexpr (is_dn<T>::value) {
+    const Spl &sp = cx.sp[s];
+    const int i = spl_interval(sp, vval(xv), cx.hint[s]);
+    auto knot = [&](int part, int idx) {
+      const double *base = part == 0 ? sp.x : (part == 1 ? sp.y : sp.S);
+      T r;
+      r = base[idx];
+#pragma unroll
+      for (int j = 0; j + 1 < T::size; ++j)
+        if (j < cx.ndir) r.d[j + 1] = cx.dtab[j * cx.stride + sp.off + part * sp.n + idx];
+      return r;
+    };
+    T x0 = knot(0, i), x1 = knot(0, i + 1);
+    T h = x1 - x0;
+    T t = (xv - x0) / h;
+    T A = 1.0 - t;
+    T B = t;
+    return A * knot(1, i) + B * knot(1, i + 1) +
+           ((A * A * A - A) * knot(2, i) + (B * B * B - B) * knot(2, i + 1)) * (h * h / 6.0);
+  } else {
+    return spl_eval(cx.sp[s], xv, cx.hint[s]);
+  }
+}
+
+// rho_Q(a) / rho_Q(1) = a^{-3(1+w0+wa)} exp(3 (a-1) wa); for dn types the
+// exponent's own tangent enters through exp((e - e0) log a)
+template <class T>
+__device__ __forceinline__ T rho_de(T a, T loga, ParT<T> w0, ParT<T> wa) {
+  const double e0 = vval(-3.0 * (1.0 + w0 + wa));
+  T r = dpow(a, e0) * dexp(3.0 * (a - 1.0) * wa);
+  if constexpr (is_dn<T>::value) r = r * dexp((-3.0 * (1.0 + w0 + wa) - e0) * loga);
+  return r;
+}
 
 template <typename T>
 struct Bg {
@@ -37,30 +103,34 @@ __device__ __forceinline__ T nu_v(const Ctx &cx, T a, int qi);
 // thermodynamics lookups of perturbations.py:176-218)
 template <typename T>
 __device__ __forceinline__ Bg<T> background(Ctx &cx, T a) {
-  const deb_cosmo_t &c = *cx.c;
+  using P = ParT<T>;
+  const P grhom = cv<T>(cx, F_GRHOM), grhog = cv<T>(cx, F_GRHOG), grhor = cv<T>(cx, F_GRHOR);
+  const P Omegam = cv<T>(cx, F_OMEGAM), Omegab = cv<T>(cx, F_OMEGAB), OmegaDE = cv<T>(cx, F_OMEGADE);
+  const P Omegak = cv<T>(cx, F_OMEGAK), Neff = cv<T>(cx, F_NEFF), Nmnu = cv<T>(cx, F_NMNU);
+  const P w0 = cv<T>(cx, F_W0), wa = cv<T>(cx, F_WA), YHe = cv<T>(cx, F_YHE), H0 = cv<T>(cx, F_H0);
   Bg<T> b;
   b.a = a;
   T loga = dlog(a);
-  b.cs2 = spl_eval(cx.sp[DEB_SPL_CS2A_LOGA], loga, cx.hint[DEB_SPL_CS2A_LOGA]) / a;
-  T xe = spl_eval(cx.sp[DEB_SPL_XE_LOGA], loga, cx.hint[DEB_SPL_XE_LOGA]);
-  T rhonu = dexp(spl_eval(cx.sp[DEB_SPL_LOGRHONU_LOGA], loga, cx.hint[DEB_SPL_LOGRHONU_LOGA]));
-  b.pb43 = (4.0 / 3.0) * (c.grhog / (c.grhom * c.Omegab)) / a;
-  b.w = c.w_DE_0 + c.w_DE_a * (1.0 - a);
-  b.rhoQ = dpow(a, -3.0 * (1.0 + c.w_DE_0 + c.w_DE_a)) * dexp(3.0 * (a - 1.0) * c.w_DE_a);
+  b.cs2 = spl_at<T>(cx, DEB_SPL_CS2A_LOGA, loga) / a;
+  T xe = spl_at<T>(cx, DEB_SPL_XE_LOGA, loga);
+  T rhonu = dexp(spl_at<T>(cx, DEB_SPL_LOGRHONU_LOGA, loga));
+  b.pb43 = (4.0 / 3.0) * (grhog / (grhom * Omegab)) / a;
+  b.w = w0 + wa * (1.0 - a);
+  b.rhoQ = rho_de<T>(a, loga, w0, wa);
   T a2 = a * a;
-  T grho = c.grhom * c.Omegam / a + (c.grhog + c.grhor * (c.Neff + c.Nmnu * rhonu)) / a2 +
-           c.grhom * c.OmegaDE * b.rhoQ * a2 + c.grhom * c.Omegak;
+  T grho = grhom * Omegam / a + (grhog + grhor * (Neff + Nmnu * rhonu)) / a2 + grhom * OmegaDE * b.rhoQ * a2 +
+           grhom * Omegak;
   b.aH = dsqrt(grho / 3.0);
-  T wprime = -c.w_DE_a * b.aH * a;
+  T wprime = -wa * b.aH * a;
   b.ca2 = b.w - wprime / 3.0 / ((1.0 + b.w) + 1e-6) / b.aH;
-  double akthom = 2.3038921003709498e-9 * (1.0 - c.YHe) * c.Omegab * c.H0 * c.H0;
+  const P akthom = 2.3038921003709498e-9 * (1.0 - YHe) * Omegab * H0 * H0;
   b.opac = xe * akthom / a2;
   return b;
 }
 
 template <typename T>
 __device__ __forceinline__ T nu_v(const Ctx &cx, T a, int qi) {
-  T aq = a * (cx.c->amnu / cx.q[qi]);
+  T aq = a * (cv<T>(cx, F_AMNU) / cx.q[qi]);
   return 1.0 / dsqrt(1.0 + aq * aq);
 }
 
@@ -75,7 +145,10 @@ struct Coef {
 
 template <typename T>
 __device__ __forceinline__ Coef<T> row_coef(uint32_t d, const Bg<T> &b, const Ctx &cx, double invtau) {
-  const deb_cosmo_t &c = *cx.c;
+  using P = ParT<T>;
+  const P grhom = cv<T>(cx, F_GRHOM), grhog = cv<T>(cx, F_GRHOG), grhor = cv<T>(cx, F_GRHOR);
+  const P Omegam = cv<T>(cx, F_OMEGAM), Omegab = cv<T>(cx, F_OMEGAB), OmegaDE = cv<T>(cx, F_OMEGADE);
+  const P Neff = cv<T>(cx, F_NEFF), Nmnu = cv<T>(cx, F_NMNU), cs2DE = cv<T>(cx, F_CS2DE);
   const double k = cx.k, k2 = k * k;
   Coef<T> r;
   r.sub = 0.0; r.diag = 0.0; r.sup = 0.0;
@@ -86,7 +159,7 @@ __device__ __forceinline__ Coef<T> row_coef(uint32_t d, const Bg<T> &b, const Ct
   const int qi = (int)rd_q(d);
   const T a = b.a;
   const T a2 = a * a;
-  const double Omegac = c.Omegam - c.Omegab;
+  const P Omegac = Omegam - Omegab;
   const T inv_aH = 1.0 / b.aH;
   switch (t) {
     case R_ETA:
@@ -96,37 +169,37 @@ __device__ __forceinline__ Coef<T> row_coef(uint32_t d, const Bg<T> &b, const Ct
     case R_DC:
       r.sup = -1.0;
       r.uh = -0.5;
-      r.ch = c.grhom * Omegac / a * inv_aH;
-      r.c1 = -a * (c.grhom * Omegac / a);
+      r.ch = grhom * Omegac / a * inv_aH;
+      r.c1 = -a * (grhom * Omegac / a);
       break;
     case R_TC:
       r.diag = -b.aH;
-      r.ceta = c.grhom * Omegac / a * (0.5 / k2);
+      r.ceta = grhom * Omegac / a * (0.5 / k2);
       break;
     case R_DB:
       r.sup = -1.0;
       r.uh = -0.5;
-      r.ch = c.grhom * c.Omegab / a * inv_aH;
-      r.c1 = -a * (c.grhom * c.Omegab / a);
+      r.ch = grhom * Omegab / a * inv_aH;
+      r.c1 = -a * (grhom * Omegab / a);
       break;
     case R_TB:
       r.sub = k2 * b.cs2;
       r.diag = -b.aH - b.pb43 * b.opac;
       r.sup = b.pb43 * b.opac;
-      r.ceta = c.grhom * c.Omegab / a * (0.5 / k2);
+      r.ceta = grhom * Omegab / a * (0.5 / k2);
       break;
     case R_DG:
       r.leaf = -4.0 / 3.0;
       r.uh = -2.0 / 3.0;
-      r.ch = c.grhog / a2 * inv_aH;
-      r.c1 = -a * (c.grhog / a2 + 3.0 * c.grhog / a2 / 3.0);
+      r.ch = grhog / a2 * inv_aH;
+      r.c1 = -a * (grhog / a2 + 3.0 * grhog / a2 / 3.0);
       break;
     case R_TG:
       r.sub = b.opac;
       r.diag = -b.opac;
       r.sup = -0.5 * k2;
       r.leaf = 0.25 * k2;
-      r.ceta = (4.0 / 3.0) * c.grhog / a2 * (0.5 / k2);
+      r.ceta = (4.0 / 3.0) * grhog / a2 * (0.5 / k2);
       break;
     case R_F2:
       r.sub = 8.0 / 15.0;
@@ -171,13 +244,13 @@ __device__ __forceinline__ Coef<T> row_coef(uint32_t d, const Bg<T> &b, const Ct
     case R_DR:
       r.sup = -4.0 / 3.0;
       r.uh = -2.0 / 3.0;
-      r.ch = c.grhor * c.Neff / a2 * inv_aH;
-      r.c1 = -a * (c.grhor * c.Neff / a2 + c.grhor * c.Neff / a2);
+      r.ch = grhor * Neff / a2 * inv_aH;
+      r.c1 = -a * (grhor * Neff / a2 + grhor * Neff / a2);
       break;
     case R_TR:
       r.sub = 0.25 * k2;
       r.sup = -0.5 * k2;
-      r.ceta = (4.0 / 3.0) * c.Neff * c.grhor / a2 * (0.5 / k2);
+      r.ceta = (4.0 / 3.0) * Neff * grhor / a2 * (0.5 / k2);
       break;
     case R_FR2:
       r.sub = 8.0 / 15.0;
@@ -198,8 +271,8 @@ __device__ __forceinline__ Coef<T> row_coef(uint32_t d, const Bg<T> &b, const Ct
       T kv = k * vq;
       r.sup = -kv;
       r.uh = cx.dlnf[qi] / 6.0;
-      r.ch = c.grhor * c.Nmnu * cx.w[qi] / vq / a2 * inv_aH;
-      r.c1 = -a * (c.grhor * c.Nmnu * cx.w[qi] / vq / a2 + c.grhor * c.Nmnu * cx.w[qi] * vq / a2);
+      r.ch = grhor * Nmnu * cx.w[qi] / vq / a2 * inv_aH;
+      r.c1 = -a * (grhor * Nmnu * cx.w[qi] / vq / a2 + grhor * Nmnu * cx.w[qi] * vq / a2);
       break;
     }
     case R_Q1: {
@@ -207,7 +280,7 @@ __device__ __forceinline__ Coef<T> row_coef(uint32_t d, const Bg<T> &b, const Ct
       T kv = k * vq;
       r.sub = kv / 3.0;
       r.sup = -2.0 * kv / 3.0;
-      r.ceta = c.Nmnu * c.grhor * k * cx.w[qi] / a2 * (0.5 / k2);
+      r.ceta = Nmnu * grhor * k * cx.w[qi] / a2 * (0.5 / k2);
       break;
     }
     case R_Q2: {
@@ -234,23 +307,23 @@ __device__ __forceinline__ Coef<T> row_coef(uint32_t d, const Bg<T> &b, const Ct
     }
     case R_DQ:
       if (!cx.lambda) {
-        const double cs2Q = c.cs2_DE;
+        const P cs2Q = cs2DE;
         T opw = 1.0 + b.w;
         r.diag = -3.0 * (cs2Q - b.w) * b.aH;
         r.sup = -opw - 9.0 * opw * (cs2Q - b.ca2) * b.aH * b.aH / k2;
         r.uh = -0.5 * opw;
-        T rq = c.grhom * c.OmegaDE * b.rhoQ * a2;
+        T rq = grhom * OmegaDE * b.rhoQ * a2;
         r.ch = rq * inv_aH;
         r.c1 = -a * (rq + 3.0 * cs2Q * rq);
       }
       break;
     case R_TQ:
       if (!cx.lambda) {
-        const double cs2Q = c.cs2_DE;
+        const P cs2Q = cs2DE;
         T opw = 1.0 + b.w;
         r.sub = cs2Q / opw * k2;
         r.diag = -(1.0 - 3.0 * cs2Q) * b.aH;
-        T rq = c.grhom * c.OmegaDE * b.rhoQ * a2;
+        T rq = grhom * OmegaDE * b.rhoQ * a2;
         r.ceta = opw * rq * (0.5 / k2);
         r.c1 = -a * (3.0 * (cs2Q - b.ca2) * (3.0 * b.aH / k2) * opw * rq);
       }

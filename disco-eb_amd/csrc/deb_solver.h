@@ -324,10 +324,11 @@ struct R5 {
                           d4 = 0.3449828624725343, d5 = -0.377417564392089818;
 };
 
-// one Rodas5Transformed step (ode_integrators_stiff.py:718-834): y1 and err = k8
-template <int NS>
-__device__ __forceinline__ void rodas5_step(Mode<NS> &m, Ctx &cx, double t0, double dt, const double (&y0)[NS],
-                            double (&y1)[NS], double (&err)[NS]) {
+// one Rodas5Transformed step (ode_integrators_stiff.py:718-834): y1 and err = k8;
+// st(i, k) sees every stage vector k_{i+1} once it is solved
+template <int NS, class Store>
+__device__ __forceinline__ void rodas5_step_s(Mode<NS> &m, Ctx &cx, double t0, double dt, const double (&y0)[NS],
+                                              double (&y1)[NS], double (&err)[NS], Store st) {
   using R = R5;
   factor<NS>(m, cx, t0, y0, dt);
   double dT[NS];
@@ -340,6 +341,7 @@ __device__ __forceinline__ void rodas5_step(Mode<NS> &m, Ctx &cx, double t0, dou
 #pragma unroll
   for (int s = 0; s < NS; ++s) k1[s] = f[s] + dt * R::d1 * dT[s];
   wsolve<NS>(m, k1);
+  st(0, k1);
   // stage 2
 #pragma unroll
   for (int s = 0; s < NS; ++s) u[s] = y0[s] + R::a21 * k1[s];
@@ -347,6 +349,7 @@ __device__ __forceinline__ void rodas5_step(Mode<NS> &m, Ctx &cx, double t0, dou
 #pragma unroll
   for (int s = 0; s < NS; ++s) k2[s] = f[s] + dt * R::d2 * dT[s] + (R::C21 * idt) * k1[s];
   wsolve<NS>(m, k2);
+  st(1, k2);
   // stage 3
 #pragma unroll
   for (int s = 0; s < NS; ++s) u[s] = y0[s] + R::a31 * k1[s] + R::a32 * k2[s];
@@ -355,6 +358,7 @@ __device__ __forceinline__ void rodas5_step(Mode<NS> &m, Ctx &cx, double t0, dou
   for (int s = 0; s < NS; ++s)
     k3[s] = f[s] + dt * R::d3 * dT[s] + ((R::C31 * idt) * k1[s] + (R::C32 * idt) * k2[s]);
   wsolve<NS>(m, k3);
+  st(2, k3);
   // stage 4
 #pragma unroll
   for (int s = 0; s < NS; ++s) u[s] = y0[s] + R::a41 * k1[s] + R::a42 * k2[s] + R::a43 * k3[s];
@@ -363,6 +367,7 @@ __device__ __forceinline__ void rodas5_step(Mode<NS> &m, Ctx &cx, double t0, dou
   for (int s = 0; s < NS; ++s)
     k4[s] = f[s] + dt * R::d4 * dT[s] + ((R::C41 * idt) * k1[s] + (R::C42 * idt) * k2[s] + (R::C43 * idt) * k3[s]);
   wsolve<NS>(m, k4);
+  st(3, k4);
   // stage 5
 #pragma unroll
   for (int s = 0; s < NS; ++s) u[s] = y0[s] + R::a51 * k1[s] + R::a52 * k2[s] + R::a53 * k3[s] + R::a54 * k4[s];
@@ -372,6 +377,7 @@ __device__ __forceinline__ void rodas5_step(Mode<NS> &m, Ctx &cx, double t0, dou
     k5[s] = f[s] + dt * R::d5 * dT[s] +
             ((R::C51 * idt) * k1[s] + (R::C52 * idt) * k2[s] + (R::C53 * idt) * k3[s] + (R::C54 * idt) * k4[s]);
   wsolve<NS>(m, k5);
+  st(4, k5);
   // stage 6
 #pragma unroll
   for (int s = 0; s < NS; ++s)
@@ -382,6 +388,7 @@ __device__ __forceinline__ void rodas5_step(Mode<NS> &m, Ctx &cx, double t0, dou
     k6[s] = f[s] + ((R::C61 * idt) * k1[s] + (R::C62 * idt) * k2[s] + (R::C63 * idt) * k3[s] +
                     (R::C64 * idt) * k4[s] + (R::C65 * idt) * k5[s]);
   wsolve<NS>(m, k6);
+  st(5, k6);
   // stage 7
 #pragma unroll
   for (int s = 0; s < NS; ++s) u[s] = u[s] + k6[s];
@@ -391,6 +398,7 @@ __device__ __forceinline__ void rodas5_step(Mode<NS> &m, Ctx &cx, double t0, dou
     k7[s] = f[s] + ((R::C71 * idt) * k1[s] + (R::C72 * idt) * k2[s] + (R::C73 * idt) * k3[s] +
                     (R::C74 * idt) * k4[s] + (R::C75 * idt) * k5[s] + (R::C76 * idt) * k6[s]);
   wsolve<NS>(m, k7);
+  st(6, k7);
   // stage 8
 #pragma unroll
   for (int s = 0; s < NS; ++s) u[s] = u[s] + k7[s];
@@ -401,8 +409,15 @@ __device__ __forceinline__ void rodas5_step(Mode<NS> &m, Ctx &cx, double t0, dou
                      (R::C84 * idt) * k4[s] + (R::C85 * idt) * k5[s] + (R::C86 * idt) * k6[s] +
                      (R::C87 * idt) * k7[s]);
   wsolve<NS>(m, err);
+  st(7, err);
 #pragma unroll
   for (int s = 0; s < NS; ++s) y1[s] = u[s] + err[s];
+}
+
+template <int NS>
+__device__ __forceinline__ void rodas5_step(Mode<NS> &m, Ctx &cx, double t0, double dt, const double (&y0)[NS],
+                                            double (&y1)[NS], double (&err)[NS]) {
+  rodas5_step_s<NS>(m, cx, t0, dt, y0, y1, err, [](int, const double (&)[NS]) {});
 }
 
 }  // namespace deb

@@ -33,6 +33,7 @@ def lib():
         L.deb_abi_version.restype = ctypes.c_int
         L.deb_last_error.restype = ctypes.c_char_p
         L.deb_nvar.argtypes = [ctypes.POINTER(DebConfig)]
+        L.deb_slots.argtypes = [ctypes.POINTER(DebConfig)]
         L.deb_workspace_bytes.argtypes = [ctypes.POINTER(DebConfig), ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                           ctypes.POINTER(ctypes.c_size_t)]
         L.deb_solve_forward.argtypes = [ctypes.POINTER(DebConfig), ctypes.c_int, _dp, _dp, _i64, _dp, _dp, _dp,
@@ -42,6 +43,19 @@ def lib():
                               _dp, _dp, _dp]
         L.deb_rodas5_step.argtypes = [ctypes.POINTER(DebConfig), _dp, _dp, _i64, _dp, _dp, _dp, ctypes.c_int, _dp,
                                       _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.deb_tape_bytes.argtypes = [ctypes.POINTER(DebConfig), ctypes.c_int, ctypes.c_int, _i64,
+                                     ctypes.POINTER(ctypes.c_size_t)]
+        L.deb_solve_forward_tape.argtypes = [ctypes.POINTER(DebConfig), ctypes.c_int, _dp, _dp, _i64, _dp, _dp, _dp,
+                                             ctypes.c_int, _dp, ctypes.c_int, _dp, _dp, _dp, _dp, _dp,
+                                             ctypes.c_size_t, _dp]
+        L.deb_solve_adjoint.argtypes = [ctypes.POINTER(DebConfig), ctypes.c_int, _dp, _dp, _i64, _dp, _dp, _dp,
+                                        ctypes.c_int, _dp, ctypes.c_int, _dp, ctypes.c_int, _dp, _dp, _dp, _dp, _dp,
+                                        _dp, _dp, ctypes.c_size_t, _dp]
+        L.deb_rhs_vjp.argtypes = [ctypes.POINTER(DebConfig), _dp, _dp, _i64, _dp, _dp, _dp, ctypes.c_int, _dp, _dp,
+                                  ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.deb_rodas5_step_vjp.argtypes = [ctypes.POINTER(DebConfig), _dp, _dp, _i64, _dp, _dp, _dp, ctypes.c_int,
+                                          _dp, _dp, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                                          _dp]
         if L.deb_abi_version() != 1:
             raise DebError("libdiscoeb_hip.so ABI version mismatch")
         _LIB = L

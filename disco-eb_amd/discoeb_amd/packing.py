@@ -21,7 +21,7 @@ SPLINE_KEYS = ("cs2a_of_loga_spline", "xe_of_loga_spline", "logrhonu_of_loga_spl
                "tau_of_a_spline", "a_of_tau_spline", "xe_of_tau_spline")
 N_OUT_FIELDS = 20
 
-STATUS_OK, STATUS_MAX_STEPS, STATUS_NONFINITE, STATUS_NOT_RUN = 0, 1, 2, -1
+STATUS_OK, STATUS_MAX_STEPS, STATUS_NONFINITE, STATUS_TAPE_FULL, STATUS_NOT_RUN = 0, 1, 2, 3, -1
 SOLVER_RODAS5 = 0
 
 

@@ -35,6 +35,7 @@ extern "C" {
 #define DEB_ABI_VERSION 1
 #define DEB_MAXNQ 16    /* maximum massive-neutrino momentum bins */
 #define DEB_NOUT_FIELDS 20
+#define DEB_MAXDIR 6    /* input tangent directions per gradient launch */
 
 /* Spline tables read on the path (param dict keys of the reference). */
 enum {
@@ -53,6 +54,8 @@ enum {
   DEB_STATUS_OK = 0,
   DEB_STATUS_MAX_STEPS = 1, /* diffrax max_steps reached before t1 */
   DEB_STATUS_NONFINITE = 2, /* NaN/inf error norm                   */
+  DEB_STATUS_TAPE_FULL = 3, /* checkpoint tape exhausted (gradient path); the
+                               forward results are complete, the tape is not */
   DEB_STATUS_NOT_RUN = -1
 };
 
@@ -94,6 +97,10 @@ const char *deb_last_error(void);
 /* Number of state variables n (nvar of evolve_one_mode, perturbations.py:739). */
 int deb_nvar(const deb_config_t *cfg);
 
+/* Lane slots per mode (the state occupies deb_slots x 64 doubles per
+ * wavefront; a checkpoint-tape chunk holds 16 such states). */
+int deb_slots(const deb_config_t *cfg);
+
 /* Bytes of device workspace deb_solve_forward needs (caller allocates). */
 int deb_workspace_bytes(const deb_config_t *cfg, int n_cosmo, int nk, int nout, size_t *bytes);
 
@@ -102,8 +109,8 @@ int deb_workspace_bytes(const deb_config_t *cfg, int n_cosmo, int nk, int nout, 
  * conditions, adaptive Rodas5 integration to max(tau_out) with outputs
  * linearly interpolated at tau_out, conversion to the 20 output fields.
  * Replaces evolve_perturbations (perturbations.py:926-997) for n_cosmo == 1
- * and its multi-cosmology generalisation for n_cosmo > 1 (one persistent
- * launch with a device work queue over all modes).
+ * and its multi-cosmology generalisation for n_cosmo > 1 (one launch, one
+ * wavefront per (cosmology, k) mode, largest k first).
  *
  *   cosmo     device  [n_cosmo] deb_cosmo_t
  *   tables    device  [n_cosmo][table_stride] doubles: for spline s the knots
@@ -143,6 +150,75 @@ int deb_rodas5_step(const deb_config_t *cfg, const deb_cosmo_t *cosmo, const dou
                     const int32_t *desc, const double *q, const double *w, int m, const int32_t *ci,
                     const double *kk, const double *t0, const double *t1, const double *y0, double *y1, double *err,
                     void *stream);
+
+/*
+ * ---- gradient path ----
+ * The reference differentiates evolve_perturbations with jax.jacfwd through
+ * diffrax's loop and Rodas5Transformed.step (nb_minimal_example.ipynb:297,
+ * nb_discoeb_autodiff.ipynb:274; the solve's DirectAdjoint at
+ * perturbations.py:765).  These entry points return the same derivative of
+ * the discrete solution in reverse mode, batched over all modes: a taped
+ * forward solve, then one backward launch.  The step-size controller's
+ * decisions carry no derivative: each accepted step keeps its fraction of
+ * [t0, t1] while the start time t0, the last output time t1, the initial
+ * state, the cosmology record and every spline knot move.
+ *
+ * Parameter derivatives are projected on ndir <= DEB_MAXDIR caller-supplied
+ * input tangents: for direction j, dcosmo[c][j] (24 doubles, the layout of
+ * deb_cosmo_t), dtables[c][j][table_stride] (every knot x, y, S) and
+ * dtau_out[c][j][nout].  grad[c][k][j] = sum over the mode's outputs of
+ * ybar . d(y_out)/d(direction j).
+ */
+
+/* Bytes of a checkpoint tape for n_states accepted states in total (sum over
+ * modes of accepted steps + 1).  A sufficient bound: sum of nsteps + n_modes. */
+int deb_tape_bytes(const deb_config_t *cfg, int n_cosmo, int nk, int64_t n_states, size_t *bytes);
+
+/*
+ * deb_solve_forward that also records every accepted state of every mode on
+ * the tape (device memory of tape_bytes, owned by the caller).  Modes whose
+ * states do not fit get DEB_STATUS_TAPE_FULL (their outputs are complete).
+ * Arguments as deb_solve_forward; return_full must be 0 for a later
+ * deb_solve_adjoint.
+ */
+int deb_solve_forward_tape(const deb_config_t *cfg, int n_cosmo, const deb_cosmo_t *cosmo, const double *tables,
+                           int64_t table_stride, const int32_t *desc, const double *q, const double *w, int nk,
+                           const double *kmodes, int nout, const double *tau_out, double *y_out, int32_t *status,
+                           int32_t *nsteps, void *tape, size_t tape_bytes, void *stream);
+
+/*
+ * Reverse sweep over a tape written by deb_solve_forward_tape with the same
+ * inputs.  ybar device [n_cosmo][nk][nout][20] cotangents of y_out; status
+ * device [n_cosmo][nk] from the taped solve (modes not OK get NaN); grad
+ * device [n_cosmo][nk][ndir]; dcosmo, dtables, dtau_out device (see above).
+ */
+int deb_solve_adjoint(const deb_config_t *cfg, int n_cosmo, const deb_cosmo_t *cosmo, const double *tables,
+                      int64_t table_stride, const int32_t *desc, const double *q, const double *w, int nk,
+                      const double *kmodes, int nout, const double *tau_out, int ndir, const double *dcosmo,
+                      const double *dtables, const double *dtau_out, const double *ybar, const int32_t *status,
+                      double *grad, const void *tape, size_t tape_bytes, void *stream);
+
+/*
+ * Operator-level test hook: reverse of model_synchronous (perturbations.py:84-371)
+ * for m tuples: ybar = (df/dy)^T fbar, taubar = fbar . df/dtau,
+ * grad[m][ndir] = fbar . df/d(direction j) at fixed (tau, y).  Tangent arrays
+ * are indexed by the tuple's cosmology index ci.  All arrays device.
+ */
+int deb_rhs_vjp(const deb_config_t *cfg, const deb_cosmo_t *cosmo, const double *tables, int64_t table_stride,
+                const int32_t *desc, const double *q, const double *w, int ndir, const double *dcosmo,
+                const double *dtables, int m, const int32_t *ci, const double *kk, const double *tau, const double *y,
+                const double *fbar, double *ybar, double *taubar, double *grad, void *stream);
+
+/*
+ * Operator-level test hook: reverse of one Rodas5Transformed step
+ * (ode_integrators_stiff.py:718-834) from (t0, y0) to t1: y0bar = (dy1/dy0)^T y1bar,
+ * t0bar, t1bar = y1bar . dy1/dt0, dy1/dt1, grad[m][ndir] = y1bar . dy1/d(direction j).
+ */
+int deb_rodas5_step_vjp(const deb_config_t *cfg, const deb_cosmo_t *cosmo, const double *tables,
+                        int64_t table_stride, const int32_t *desc, const double *q, const double *w, int ndir,
+                        const double *dcosmo, const double *dtables, int m, const int32_t *ci, const double *kk,
+                        const double *t0, const double *t1, const double *y0, const double *y1bar, double *y0bar,
+                        double *t0bar, double *t1bar, double *grad, void *stream);
 
 #ifdef __cplusplus
 }

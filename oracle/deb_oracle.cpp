@@ -730,19 +730,11 @@ int evolve_mode(const Model &m, const SolverCfg &s, const double *tau_out, int n
 }
 
 // directional derivative of the 20 output fields along one tangent of the
-// inputs, with the accepted steps of the plain solve held fixed as fractions
-// of [t0, t1] (see the file header)
-int evolve_tangent(const Model &m, const ModelT<Dd> &md, const SolverCfg &s, const double *tau_out,
-                   const double *dtau_out, int nout, double *out, double *dout, int *naccept) {
-  const int n = m.nvar();
-  std::vector<double> times, plain((size_t)nout * 20);
-  int nsteps = 0, nacc = 0;
-  double ts0 = 0.0;
-  SolverCfg s20 = s;
-  s20.return_full = 0;
-  int st = evolve_mode(m, s20, tau_out, nout, plain.data(), &nsteps, &nacc, &ts0, &times);
-  *naccept = nacc;
-  if (st != DEB_STATUS_OK) return st;
+// inputs, replaying the accepted step end times `times` (t0 first) with each
+// step held fixed as a fraction of [t0, t1] (see the file header)
+void replay_tangent(const ModelT<Dd> &md, const std::vector<double> &times, const double *tau_out,
+                    const double *dtau_out, int nout, double *out, double *dout) {
+  const int n = md.nvar();
   std::vector<Dd> tout(nout);
   for (int i = 0; i < nout; ++i) tout[i] = Dd(tau_out[i], dtau_out[i]);
   Dd tmin = tout[0], tmax = tout[0];
@@ -759,7 +751,7 @@ int evolve_tangent(const Model &m, const ModelT<Dd> &md, const SolverCfg &s, con
   };
   Work<Dd> w(n);
   std::vector<Dd> y(n), y1(n), err(n), ys(n), o(20);
-  ics<Dd>(md, T0, y.data());
+  ics<Dd>(md, time_of(0), y.data());
   int iout = 0;
   for (size_t st_i = 0; st_i + 1 < times.size(); ++st_i) {
     const Dd ta = time_of(st_i), tb = time_of(st_i + 1);
@@ -776,6 +768,19 @@ int evolve_tangent(const Model &m, const ModelT<Dd> &md, const SolverCfg &s, con
     }
     y.swap(y1);
   }
+}
+
+int evolve_tangent(const Model &m, const ModelT<Dd> &md, const SolverCfg &s, const double *tau_out,
+                   const double *dtau_out, int nout, double *out, double *dout, int *naccept) {
+  std::vector<double> times, plain((size_t)nout * 20);
+  int nsteps = 0, nacc = 0;
+  double ts0 = 0.0;
+  SolverCfg s20 = s;
+  s20.return_full = 0;
+  int st = evolve_mode(m, s20, tau_out, nout, plain.data(), &nsteps, &nacc, &ts0, &times);
+  *naccept = nacc;
+  if (st != DEB_STATUS_OK) return st;
+  replay_tangent(md, times, tau_out, dtau_out, nout, out, dout);
   return DEB_STATUS_OK;
 }
 
@@ -1061,6 +1066,19 @@ int oracle_evolve_tangent(const deb_config_t *cfg, const deb_cosmo_t *c, const d
     for (int t = 0; t < nthreads; ++t) th.emplace_back(worker);
     for (auto &t : th) t.join();
   }
+  return 0;
+}
+
+// tangent of the outputs along one input tangent for a given sequence of
+// accepted step end times (times[0] = start time); see replay_tangent
+int oracle_replay_tangent(const deb_config_t *cfg, const deb_cosmo_t *c, const double *tables,
+                          const deb_cosmo_t *dc, const double *dtables, const int32_t *desc, const double *q,
+                          const double *w, double k, int ntimes, const double *times, int nout,
+                          const double *tau_out, const double *dtau_out, double *y_out, double *dy_out) {
+  std::vector<Dd> store;
+  ModelT<Dd> md = make_model_tangent(c, tables, dc, dtables, desc, cfg, q, w, k, store);
+  std::vector<double> tv(times, times + ntimes);
+  replay_tangent(md, tv, tau_out, dtau_out, nout, y_out, dy_out);
   return 0;
 }
 

@@ -172,6 +172,18 @@ class Problem:
                                     ctypes.c_int(nthreads))
         return dict(y=y, dy=dy, status=st, naccept=na)
 
+    def replay_tangent(self, k, times, tau_out, dtau_out, dcos, dtab):
+        """Output fields and their tangent for the accepted step end times
+        ``times`` (times[0] = start time) of some other solve of this mode."""
+        times = np.ascontiguousarray(times, dtype=np.float64)
+        tau_out = np.ascontiguousarray(np.atleast_1d(tau_out), dtype=np.float64)
+        dtau_out = np.ascontiguousarray(np.atleast_1d(dtau_out), dtype=np.float64)
+        nout = tau_out.size
+        y, dy = np.zeros((nout, 20)), np.zeros((nout, 20))
+        lib().oracle_replay_tangent(*self._tan(dcos, dtab), ctypes.c_double(k), ctypes.c_int(times.size), _p(times),
+                                    ctypes.c_int(nout), _p(tau_out), _p(dtau_out), _p(y), _p(dy))
+        return y, dy
+
     def perturbed(self, eps, dcos, dtab):
         """A copy whose inputs are moved by eps along (dcos, dtab)."""
         other = Problem.__new__(Problem)
