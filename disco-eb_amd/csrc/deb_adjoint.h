@@ -261,6 +261,86 @@ __device__ __forceinline__ void tapply(Ctx &cx, const uint32_t (&rd)[NS], T a, d
   for (int s = 0; s < NS; ++s) out[s] = out[s] + cc[0][s] * su[0] + cc[1][s] * su[1] + cc[2][s] * su[2] + cc[3][s] * wahp;
 }
 
+// Values of the functionals of Y and of the "where they enter" sums of w that
+// the first-order contraction below needs, plus (optionally) the transposed
+// right-hand side A(a)^T w -- one plain-double pass over the rows.
+struct RowSums {
+  double H, E, Pi, F1;  // sum c_{h,eta,pi,1} Y
+  double sh, se, sp;    // sum u_{h,eta,pi} w
+};
+
+template <int NS, bool TAPPLY>
+__device__ __forceinline__ RowSums coef_pass(Ctx &cx, const uint32_t (&rd)[NS], double a, double invtau,
+                                             const double (&w)[NS], const double (&Y)[NS], double (&out)[NS]) {
+  const int lane = lane_id();
+  Bg<double> b = background<double>(cx, a);
+  double part[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  double cc[4][NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    Coef<double> r = row_coef<double>(rd[s], b, cx, invtau);
+    part[0] += r.ch * Y[s];
+    part[1] += r.ceta * Y[s];
+    part[2] += r.cpi * Y[s];
+    part[3] += r.c1 * Y[s];
+    part[4] += r.uh * w[s];
+    part[5] += r.ueta * w[s];
+    part[6] += r.upi * w[s];
+    if constexpr (TAPPLY) {
+      out[s] = r.diag * w[s] + from_next(r.sub * w[s]) + from_prev(r.sup * w[s]);
+      if (s == 0) {
+        const double lw = r.leaf * w[0];
+        const double l_dg = readlane(lw, L_DG), l_tg = readlane(lw, L_TG);
+        if (lane == L_TG) out[0] += l_dg;
+        if (lane == L_DG) out[0] += l_tg;
+      }
+      cc[0][s] = r.ch;
+      cc[1][s] = r.ceta;
+      cc[2][s] = r.cpi;
+      cc[3][s] = r.c1;
+    }
+  }
+  wave_sum<7>(part);
+  RowSums rs{part[0], part[1], part[2], part[3], part[4], part[5], part[6]};
+  if constexpr (TAPPLY) {
+    const double wahp = readlane(w[0], L_AHP);
+#pragma unroll
+    for (int s = 0; s < NS; ++s) out[s] += cc[0][s] * rs.sh + cc[1][s] * rs.se + cc[2][s] * rs.sp + cc[3][s] * wahp;
+  }
+  return rs;
+}
+
+// First-order contraction: the derivative components of the returned T equal
+// those of sum_rows w_row f_row(a, Y) (the value component does not).  The
+// functionals enter through their double values (rs), which is exact for
+// first derivatives: d(u H) = du H + u dH and sum_rows w u dH = s_h dH.
+// Coefficients are streamed through row_visit, so no row is held whole.
+template <int NS, class T>
+__device__ __forceinline__ T contract1(Ctx &cx, const uint32_t (&rd)[NS], T a, double invtau, const double (&Y)[NS],
+                                       const double (&w)[NS], const RowSums &rs) {
+  const int lane = lane_id();
+  Bg<T> b = background<T>(cx, a);
+  T acc = tconst<T>(0.0);
+  const double ydg = readlane(Y[0], L_DG), ytg = readlane(Y[0], L_TG);
+  const double wahp = readlane(w[0], L_AHP);
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    const double ym = from_prev(Y[s]), yp = from_next(Y[s]);
+    const double lp = (s == 0) ? (lane == L_DG ? ytg : (lane == L_TG ? ydg : 0.0)) : 0.0;
+    // multiplier of each coefficient field in the contraction
+    const double mul[11] = {w[s] * ym,    w[s] * Y[s],    w[s] * yp,       w[s] * rs.H, w[s] * rs.E,
+                            w[s] * rs.Pi, Y[s] * rs.sh,   Y[s] * rs.se,    Y[s] * rs.sp, Y[s] * wahp,
+                            w[s] * lp};
+    row_visit<T>(rd[s], b, cx, invtau, [&](auto f, const auto &v) __attribute__((always_inline)) {
+      acc = acc + v * mul[decltype(f)::value];
+    });
+  }
+  if (lane == L_A) acc = acc + b.aH * b.a * w[0];
+  T tot[1] = {acc};
+  t_wave_sum<T, 1>(tot);
+  return tot[0];
+}
+
 // per-lane multiplier of the explicit tau dependence (hierarchy truncations):
 // d f_row / d tau = trunc_row / tau^2 * y_row with trunc_row = l + 1
 template <int NS>
@@ -372,8 +452,8 @@ __device__ __forceinline__ void rodas5_step_vjp(Mode<NS> &m, Ctx &cx, double t0,
     double ub[NS];
     {
       const double ai = readlane(u[0], L_A);
-      tapply<NS, double>(cx, m.rd, ai, iti, rb, ub);
-      TD g = contract<NS, TD>(cx, m.rd, seed_td(ai, 1.0), iti, u, rb);
+      const RowSums rs = coef_pass<NS, true>(cx, m.rd, ai, iti, rb, u, ub);
+      TD g = contract1<NS, TD>(cx, m.rd, seed_td(ai, 1.0), iti, u, rb, rs);
       if (lane == L_A) ub[0] += g.d[0];
 #pragma unroll
       for (int j = 0; j < MAXDIR; ++j) th_acc[j] += g.d[j + 1];
@@ -391,7 +471,9 @@ __device__ __forceinline__ void rodas5_step_vjp(Mode<NS> &m, Ctx &cx, double t0,
 #pragma unroll
       for (int s = 0; s < NS; ++s) wr[s] = rb[s];
       if (lane == L_A) wr[0] = 0.0;
-      TD g = contract<NS, TD>(cx, m.rd, seed_td(a0, 1.0), it0, ki, wr);
+      double unused[NS];
+      const RowSums rs = coef_pass<NS, false>(cx, m.rd, a0, it0, wr, ki, unused);
+      TD g = contract1<NS, TD>(cx, m.rd, seed_td(a0, 1.0), it0, ki, wr, rs);
       a0b += g.d[0];
 #pragma unroll
       for (int j = 0; j < MAXDIR; ++j) th_acc[j] += g.d[j + 1];

@@ -80,8 +80,8 @@ __global__ void __launch_bounds__(WAVE) deb_rhs_vjp_kernel(const Params<NS> P, c
   }
   const double tau = H.t0[item], it = 1.0 / tau;
   const double a = readlane(y[0], L_A);
-  tapply<NS, double>(cx, rd, a, it, w, yb);
-  TD g = contract<NS, TD>(cx, rd, seed_td(a, 1.0), it, y, w);
+  const RowSums rs = coef_pass<NS, true>(cx, rd, a, it, w, y, yb);
+  TD g = contract1<NS, TD>(cx, rd, seed_td(a, 1.0), it, y, w, rs);
   if (lane == L_A) yb[0] += g.d[0];
   trunc_weights<NS>(rd, tw);
   double pt = 0.0;

@@ -296,11 +296,6 @@ __device__ __forceinline__ double dpp_f64(double x) {
   hi = __builtin_amdgcn_update_dpp(0, hi, CTRL, 0xf, 0xf, BOUND_ZERO);
   return __hiloint2double(hi, lo);
 }
-// value of lane i-1 (lane 0 receives 0)
-__device__ __forceinline__ double from_prev(double x) { return dpp_f64<0x138, false>(x); }
-// value of lane i+1 (lane 63 receives 0)
-__device__ __forceinline__ double from_next(double x) { return dpp_f64<0x130, false>(x); }
-
 // value of an arbitrary lane (byte address = 4 * lane)
 __device__ __forceinline__ double from_lane(double x, int src) {
   int lo = __double2loint(x), hi = __double2hiint(x);
@@ -308,6 +303,23 @@ __device__ __forceinline__ double from_lane(double x, int src) {
   hi = __builtin_amdgcn_ds_bpermute(src << 2, hi);
   return __hiloint2double(hi, lo);
 }
+#ifdef DEB_NO_DPP
+__device__ __forceinline__ double from_prev(double x) {
+  const int l = __lane_id();
+  double v = from_lane(x, l > 0 ? l - 1 : 0);
+  return l > 0 ? v : 0.0;
+}
+__device__ __forceinline__ double from_next(double x) {
+  const int l = __lane_id();
+  double v = from_lane(x, l < 63 ? l + 1 : 63);
+  return l < 63 ? v : 0.0;
+}
+#else
+// value of lane i-1 (lane 0 receives 0)
+__device__ __forceinline__ double from_prev(double x) { return dpp_f64<0x138, false>(x); }
+// value of lane i+1 (lane 63 receives 0)
+__device__ __forceinline__ double from_next(double x) { return dpp_f64<0x130, false>(x); }
+#endif
 
 __device__ __forceinline__ double readlane(double x, int lane) {
   int lo = __builtin_amdgcn_readlane(__double2loint(x), lane);
@@ -324,6 +336,16 @@ __device__ __forceinline__ double readfirst(double x) {
 // half-row and row mirrors, then xor 16 and xor 32 (ds_bpermute).  Every
 // partial sum pairs two equal-sized groups, so each lane ends with the same
 // bits; the results stay in vector registers (no scalar-register pressure).
+#ifdef DEB_NO_DPP
+__device__ __forceinline__ double row_sum16(double x) {
+  const int l = __lane_id();
+  x += from_lane(x, l ^ 1);
+  x += from_lane(x, l ^ 2);
+  x += from_lane(x, l ^ 4);
+  x += from_lane(x, l ^ 8);
+  return x;
+}
+#else
 __device__ __forceinline__ double row_sum16(double x) {
   x += dpp_f64<0xB1, true>(x);   // quad_perm [1,0,3,2]
   x += dpp_f64<0x4E, true>(x);   // quad_perm [2,3,0,1]
@@ -331,6 +353,7 @@ __device__ __forceinline__ double row_sum16(double x) {
   x += dpp_f64<0x140, true>(x);  // row_mirror
   return x;
 }
+#endif
 template <int N>
 __device__ __forceinline__ void wave_sum(double (&v)[N]) {
   const int lane = lane_id();

@@ -8,6 +8,8 @@
 // f_1 (:252) obtained as wave reductions sum(c * y).
 #pragma once
 
+#include <type_traits>
+
 #include "deb_device.h"
 
 namespace deb {
@@ -143,17 +145,16 @@ struct Coef {
   T leaf;             // coupling to the leaf partner (delta_gamma <-> theta_gamma)
 };
 
-template <typename T>
-__device__ __forceinline__ Coef<T> row_coef(uint32_t d, const Bg<T> &b, const Ctx &cx, double invtau) {
+// coefficient fields of a row
+enum : int { C_SUB = 0, C_DIAG, C_SUP, C_UH, C_UETA, C_UPI, C_CH, C_CETA, C_CPI, C_C1, C_LEAF };
+
+template <typename T, class Sink>
+__device__ __forceinline__ void row_visit(uint32_t d, const Bg<T> &b, const Ctx &cx, double invtau, Sink sink) {
   using P = ParT<T>;
   const P grhom = cv<T>(cx, F_GRHOM), grhog = cv<T>(cx, F_GRHOG), grhor = cv<T>(cx, F_GRHOR);
   const P Omegam = cv<T>(cx, F_OMEGAM), Omegab = cv<T>(cx, F_OMEGAB), OmegaDE = cv<T>(cx, F_OMEGADE);
   const P Neff = cv<T>(cx, F_NEFF), Nmnu = cv<T>(cx, F_NMNU), cs2DE = cv<T>(cx, F_CS2DE);
   const double k = cx.k, k2 = k * k;
-  Coef<T> r;
-  r.sub = 0.0; r.diag = 0.0; r.sup = 0.0;
-  r.uh = 0.0; r.ueta = 0.0; r.upi = 0.0;
-  r.ch = 0.0; r.ceta = 0.0; r.cpi = 0.0; r.c1 = 0.0; r.leaf = 0.0;
   const uint32_t t = rd_type(d);
   const double l = (double)rd_l(d);
   const int qi = (int)rd_q(d);
@@ -163,174 +164,190 @@ __device__ __forceinline__ Coef<T> row_coef(uint32_t d, const Bg<T> &b, const Ct
   const T inv_aH = 1.0 / b.aH;
   switch (t) {
     case R_ETA:
-      r.ueta = 1.0;
-      r.ch = 2.0 * k2 * inv_aH;
+      sink(std::integral_constant<int, C_UETA>{}, 1.0);
+      sink(std::integral_constant<int, C_CH>{}, 2.0 * k2 * inv_aH);
       break;
     case R_DC:
-      r.sup = -1.0;
-      r.uh = -0.5;
-      r.ch = grhom * Omegac / a * inv_aH;
-      r.c1 = -a * (grhom * Omegac / a);
+      sink(std::integral_constant<int, C_SUP>{}, -1.0);
+      sink(std::integral_constant<int, C_UH>{}, -0.5);
+      sink(std::integral_constant<int, C_CH>{}, grhom * Omegac / a * inv_aH);
+      sink(std::integral_constant<int, C_C1>{}, -a * (grhom * Omegac / a));
       break;
     case R_TC:
-      r.diag = -b.aH;
-      r.ceta = grhom * Omegac / a * (0.5 / k2);
+      sink(std::integral_constant<int, C_DIAG>{}, -b.aH);
+      sink(std::integral_constant<int, C_CETA>{}, grhom * Omegac / a * (0.5 / k2));
       break;
     case R_DB:
-      r.sup = -1.0;
-      r.uh = -0.5;
-      r.ch = grhom * Omegab / a * inv_aH;
-      r.c1 = -a * (grhom * Omegab / a);
+      sink(std::integral_constant<int, C_SUP>{}, -1.0);
+      sink(std::integral_constant<int, C_UH>{}, -0.5);
+      sink(std::integral_constant<int, C_CH>{}, grhom * Omegab / a * inv_aH);
+      sink(std::integral_constant<int, C_C1>{}, -a * (grhom * Omegab / a));
       break;
     case R_TB:
-      r.sub = k2 * b.cs2;
-      r.diag = -b.aH - b.pb43 * b.opac;
-      r.sup = b.pb43 * b.opac;
-      r.ceta = grhom * Omegab / a * (0.5 / k2);
+      sink(std::integral_constant<int, C_SUB>{}, k2 * b.cs2);
+      sink(std::integral_constant<int, C_DIAG>{}, -b.aH - b.pb43 * b.opac);
+      sink(std::integral_constant<int, C_SUP>{}, b.pb43 * b.opac);
+      sink(std::integral_constant<int, C_CETA>{}, grhom * Omegab / a * (0.5 / k2));
       break;
     case R_DG:
-      r.leaf = -4.0 / 3.0;
-      r.uh = -2.0 / 3.0;
-      r.ch = grhog / a2 * inv_aH;
-      r.c1 = -a * (grhog / a2 + 3.0 * grhog / a2 / 3.0);
+      sink(std::integral_constant<int, C_LEAF>{}, -4.0 / 3.0);
+      sink(std::integral_constant<int, C_UH>{}, -2.0 / 3.0);
+      sink(std::integral_constant<int, C_CH>{}, grhog / a2 * inv_aH);
+      sink(std::integral_constant<int, C_C1>{}, -a * (grhog / a2 + 3.0 * grhog / a2 / 3.0));
       break;
     case R_TG:
-      r.sub = b.opac;
-      r.diag = -b.opac;
-      r.sup = -0.5 * k2;
-      r.leaf = 0.25 * k2;
-      r.ceta = (4.0 / 3.0) * grhog / a2 * (0.5 / k2);
+      sink(std::integral_constant<int, C_SUB>{}, b.opac);
+      sink(std::integral_constant<int, C_DIAG>{}, -b.opac);
+      sink(std::integral_constant<int, C_SUP>{}, -0.5 * k2);
+      sink(std::integral_constant<int, C_LEAF>{}, 0.25 * k2);
+      sink(std::integral_constant<int, C_CETA>{}, (4.0 / 3.0) * grhog / a2 * (0.5 / k2));
       break;
     case R_F2:
-      r.sub = 8.0 / 15.0;
-      r.diag = -b.opac;
-      r.sup = -0.6 * k;
-      r.uh = 4.0 / 15.0;
-      r.ueta = 8.0 / 5.0;
-      r.upi = 0.1 * b.opac;
-      r.cpi = 1.0;
+      sink(std::integral_constant<int, C_SUB>{}, 8.0 / 15.0);
+      sink(std::integral_constant<int, C_DIAG>{}, -b.opac);
+      sink(std::integral_constant<int, C_SUP>{}, -0.6 * k);
+      sink(std::integral_constant<int, C_UH>{}, 4.0 / 15.0);
+      sink(std::integral_constant<int, C_UETA>{}, 8.0 / 5.0);
+      sink(std::integral_constant<int, C_UPI>{}, 0.1 * b.opac);
+      sink(std::integral_constant<int, C_CPI>{}, 1.0);
       break;
     case R_FL:
-      r.sub = k * l / (2.0 * l + 1.0);
-      r.diag = -b.opac;
-      r.sup = -k * (l + 1.0) / (2.0 * l + 1.0);
+      sink(std::integral_constant<int, C_SUB>{}, k * l / (2.0 * l + 1.0));
+      sink(std::integral_constant<int, C_DIAG>{}, -b.opac);
+      sink(std::integral_constant<int, C_SUP>{}, -k * (l + 1.0) / (2.0 * l + 1.0));
       break;
     case R_FLMAX:
-      r.sub = k;
-      r.diag = -(l + 1.0) * invtau - b.opac;
+      sink(std::integral_constant<int, C_SUB>{}, k);
+      sink(std::integral_constant<int, C_DIAG>{}, -(l + 1.0) * invtau - b.opac);
       break;
     case R_G0:
-      r.diag = -b.opac;
-      r.sup = -k;
-      r.upi = 0.5 * b.opac;
-      r.cpi = 1.0;
+      sink(std::integral_constant<int, C_DIAG>{}, -b.opac);
+      sink(std::integral_constant<int, C_SUP>{}, -k);
+      sink(std::integral_constant<int, C_UPI>{}, 0.5 * b.opac);
+      sink(std::integral_constant<int, C_CPI>{}, 1.0);
       break;
     case R_GL:
-      r.sub = k * l / (2.0 * l + 1.0);
-      r.diag = -b.opac;
-      r.sup = -k * (l + 1.0) / (2.0 * l + 1.0);
+      sink(std::integral_constant<int, C_SUB>{}, k * l / (2.0 * l + 1.0));
+      sink(std::integral_constant<int, C_DIAG>{}, -b.opac);
+      sink(std::integral_constant<int, C_SUP>{}, -k * (l + 1.0) / (2.0 * l + 1.0));
       break;
     case R_G2:
-      r.sub = k * 2.0 / 5.0;
-      r.diag = -b.opac;
-      r.sup = -k * 3.0 / 5.0;
-      r.upi = 0.1 * b.opac;
-      r.cpi = 1.0;
+      sink(std::integral_constant<int, C_SUB>{}, k * 2.0 / 5.0);
+      sink(std::integral_constant<int, C_DIAG>{}, -b.opac);
+      sink(std::integral_constant<int, C_SUP>{}, -k * 3.0 / 5.0);
+      sink(std::integral_constant<int, C_UPI>{}, 0.1 * b.opac);
+      sink(std::integral_constant<int, C_CPI>{}, 1.0);
       break;
     case R_GLMAX:
-      r.sub = k;
-      r.diag = -(l + 1.0) * invtau - b.opac;
+      sink(std::integral_constant<int, C_SUB>{}, k);
+      sink(std::integral_constant<int, C_DIAG>{}, -(l + 1.0) * invtau - b.opac);
       break;
     case R_DR:
-      r.sup = -4.0 / 3.0;
-      r.uh = -2.0 / 3.0;
-      r.ch = grhor * Neff / a2 * inv_aH;
-      r.c1 = -a * (grhor * Neff / a2 + grhor * Neff / a2);
+      sink(std::integral_constant<int, C_SUP>{}, -4.0 / 3.0);
+      sink(std::integral_constant<int, C_UH>{}, -2.0 / 3.0);
+      sink(std::integral_constant<int, C_CH>{}, grhor * Neff / a2 * inv_aH);
+      sink(std::integral_constant<int, C_C1>{}, -a * (grhor * Neff / a2 + grhor * Neff / a2));
       break;
     case R_TR:
-      r.sub = 0.25 * k2;
-      r.sup = -0.5 * k2;
-      r.ceta = (4.0 / 3.0) * Neff * grhor / a2 * (0.5 / k2);
+      sink(std::integral_constant<int, C_SUB>{}, 0.25 * k2);
+      sink(std::integral_constant<int, C_SUP>{}, -0.5 * k2);
+      sink(std::integral_constant<int, C_CETA>{}, (4.0 / 3.0) * Neff * grhor / a2 * (0.5 / k2));
       break;
     case R_FR2:
-      r.sub = 8.0 / 15.0;
-      r.sup = -0.6 * k;
-      r.uh = 4.0 / 15.0;
-      r.ueta = 8.0 / 5.0;
+      sink(std::integral_constant<int, C_SUB>{}, 8.0 / 15.0);
+      sink(std::integral_constant<int, C_SUP>{}, -0.6 * k);
+      sink(std::integral_constant<int, C_UH>{}, 4.0 / 15.0);
+      sink(std::integral_constant<int, C_UETA>{}, 8.0 / 5.0);
       break;
     case R_FRL:
-      r.sub = k * l / (2.0 * l + 1.0);
-      r.sup = -k * (l + 1.0) / (2.0 * l + 1.0);
+      sink(std::integral_constant<int, C_SUB>{}, k * l / (2.0 * l + 1.0));
+      sink(std::integral_constant<int, C_SUP>{}, -k * (l + 1.0) / (2.0 * l + 1.0));
       break;
     case R_FRLMAX:
-      r.sub = k;
-      r.diag = -(l + 1.0) * invtau;
+      sink(std::integral_constant<int, C_SUB>{}, k);
+      sink(std::integral_constant<int, C_DIAG>{}, -(l + 1.0) * invtau);
       break;
     case R_Q0: {
       T vq = nu_v<T>(cx, a, qi);
       T kv = k * vq;
-      r.sup = -kv;
-      r.uh = cx.dlnf[qi] / 6.0;
-      r.ch = grhor * Nmnu * cx.w[qi] / vq / a2 * inv_aH;
-      r.c1 = -a * (grhor * Nmnu * cx.w[qi] / vq / a2 + grhor * Nmnu * cx.w[qi] * vq / a2);
+      sink(std::integral_constant<int, C_SUP>{}, -kv);
+      sink(std::integral_constant<int, C_UH>{}, cx.dlnf[qi] / 6.0);
+      sink(std::integral_constant<int, C_CH>{}, grhor * Nmnu * cx.w[qi] / vq / a2 * inv_aH);
+      sink(std::integral_constant<int, C_C1>{}, -a * (grhor * Nmnu * cx.w[qi] / vq / a2 + grhor * Nmnu * cx.w[qi] * vq / a2));
       break;
     }
     case R_Q1: {
       T vq = nu_v<T>(cx, a, qi);
       T kv = k * vq;
-      r.sub = kv / 3.0;
-      r.sup = -2.0 * kv / 3.0;
-      r.ceta = Nmnu * grhor * k * cx.w[qi] / a2 * (0.5 / k2);
+      sink(std::integral_constant<int, C_SUB>{}, kv / 3.0);
+      sink(std::integral_constant<int, C_SUP>{}, -2.0 * kv / 3.0);
+      sink(std::integral_constant<int, C_CETA>{}, Nmnu * grhor * k * cx.w[qi] / a2 * (0.5 / k2));
       break;
     }
     case R_Q2: {
       T vq = nu_v<T>(cx, a, qi);
       T kv = k * vq;
-      r.sub = 2.0 * kv / 5.0;
-      r.sup = -3.0 * kv / 5.0;
-      r.uh = -cx.dlnf[qi] / 15.0;
-      r.ueta = -0.4 * cx.dlnf[qi];
+      sink(std::integral_constant<int, C_SUB>{}, 2.0 * kv / 5.0);
+      sink(std::integral_constant<int, C_SUP>{}, -3.0 * kv / 5.0);
+      sink(std::integral_constant<int, C_UH>{}, -cx.dlnf[qi] / 15.0);
+      sink(std::integral_constant<int, C_UETA>{}, -0.4 * cx.dlnf[qi]);
       break;
     }
     case R_QL: {
       T vq = nu_v<T>(cx, a, qi);
       T kv = k * vq;
-      r.sub = kv * (l / (2.0 * l + 1.0));
-      r.sup = -kv * ((l + 1.0) / (2.0 * l + 1.0));
+      sink(std::integral_constant<int, C_SUB>{}, kv * (l / (2.0 * l + 1.0)));
+      sink(std::integral_constant<int, C_SUP>{}, -kv * ((l + 1.0) / (2.0 * l + 1.0)));
       break;
     }
     case R_QLMAX: {
       T vq = nu_v<T>(cx, a, qi);
-      r.sub = k * vq;
-      r.diag = -(l + 1.0) * invtau;
+      sink(std::integral_constant<int, C_SUB>{}, k * vq);
+      sink(std::integral_constant<int, C_DIAG>{}, -(l + 1.0) * invtau);
       break;
     }
     case R_DQ:
       if (!cx.lambda) {
         const P cs2Q = cs2DE;
         T opw = 1.0 + b.w;
-        r.diag = -3.0 * (cs2Q - b.w) * b.aH;
-        r.sup = -opw - 9.0 * opw * (cs2Q - b.ca2) * b.aH * b.aH / k2;
-        r.uh = -0.5 * opw;
+        sink(std::integral_constant<int, C_DIAG>{}, -3.0 * (cs2Q - b.w) * b.aH);
+        sink(std::integral_constant<int, C_SUP>{}, -opw - 9.0 * opw * (cs2Q - b.ca2) * b.aH * b.aH / k2);
+        sink(std::integral_constant<int, C_UH>{}, -0.5 * opw);
         T rq = grhom * OmegaDE * b.rhoQ * a2;
-        r.ch = rq * inv_aH;
-        r.c1 = -a * (rq + 3.0 * cs2Q * rq);
+        sink(std::integral_constant<int, C_CH>{}, rq * inv_aH);
+        sink(std::integral_constant<int, C_C1>{}, -a * (rq + 3.0 * cs2Q * rq));
       }
       break;
     case R_TQ:
       if (!cx.lambda) {
         const P cs2Q = cs2DE;
         T opw = 1.0 + b.w;
-        r.sub = cs2Q / opw * k2;
-        r.diag = -(1.0 - 3.0 * cs2Q) * b.aH;
+        sink(std::integral_constant<int, C_SUB>{}, cs2Q / opw * k2);
+        sink(std::integral_constant<int, C_DIAG>{}, -(1.0 - 3.0 * cs2Q) * b.aH);
         T rq = grhom * OmegaDE * b.rhoQ * a2;
-        r.ceta = opw * rq * (0.5 / k2);
-        r.c1 = -a * (3.0 * (cs2Q - b.ca2) * (3.0 * b.aH / k2) * opw * rq);
+        sink(std::integral_constant<int, C_CETA>{}, opw * rq * (0.5 / k2));
+        sink(std::integral_constant<int, C_C1>{}, -a * (3.0 * (cs2Q - b.ca2) * (3.0 * b.aH / k2) * opw * rq));
       }
       break;
     default:
       break;
   }
+}
+
+
+// all coefficients of a row (fields not set by the row are zero)
+template <typename T>
+__device__ __forceinline__ Coef<T> row_coef(uint32_t d, const Bg<T> &b, const Ctx &cx, double invtau) {
+  T v[11];
+#pragma unroll
+  for (int i = 0; i < 11; ++i) v[i] = 0.0;
+  row_visit<T>(d, b, cx, invtau, [&](auto f, const auto &x) __attribute__((always_inline)) {
+    v[decltype(f)::value] = v[decltype(f)::value] + x;
+  });
+  Coef<T> r;
+  r.sub = v[C_SUB]; r.diag = v[C_DIAG]; r.sup = v[C_SUP];
+  r.uh = v[C_UH]; r.ueta = v[C_UETA]; r.upi = v[C_UPI];
+  r.ch = v[C_CH]; r.ceta = v[C_CETA]; r.cpi = v[C_CPI]; r.c1 = v[C_C1]; r.leaf = v[C_LEAF];
   return r;
 }
 
