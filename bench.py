@@ -78,6 +78,10 @@ def flops_per_reverse_step(cfg) -> float:
     return flops_per_step(cfg) + 8 * per_stage + 2 * f_rhs + f_d16
 
 
+def progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def make_inputs(theta, cfg, dev, nk, with_tangents):
     import torch
     from discoeb_amd import packing, tangents
@@ -162,6 +166,7 @@ def main():
     pref = prefactor(th_t, kk)
     dth = torch.from_numpy(np.stack([th0[:, j] * FD_REL for j in FD_PARAMS], axis=1)).to(dev)
     setup_s = time.time() - t_setup
+    progress(f"setup done ({setup_s:.1f} s): {B} solves x {nk} k")
     last = {}
     n_states = [nc * nk * 2048]
 
@@ -194,17 +199,20 @@ def main():
                 last.update(P=P)
             last.update(st=st, ns=ns, ev={"forward": ev})
 
-    for _ in range(args.warmup):
+    for i in range(args.warmup):
+        tw = time.perf_counter()
         step()
         torch.cuda.synchronize()
+        progress(f"warmup step {i + 1}/{args.warmup} done ({time.perf_counter() - tw:.1f} s)")
         if args.arm == "adjoint":
             n_states[0] = int(last["ns"].sum().item()) + nc * nk  # exact bound for the timed steps
     kern = {}
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for i in range(args.steps):
         step()
         torch.cuda.synchronize()
+        progress(f"timed step {i + 1}/{args.steps} done")
         for name, (e0, e1) in last["ev"].items():
             kern.setdefault(name, []).append(e0.elapsed_time(e1))
     torch.cuda.synchronize()
@@ -247,8 +255,10 @@ def main():
     if args.arm == "adjoint":
         out["tape_bytes"] = last["tape"]
         if args.plain_sample > 0:
+            progress("timing the plain baseline")
             out["plain_baseline"] = plain_baseline(cfg, th0[:args.plain_sample], nk, dev, value)
     if not args.no_cpu_baseline:
+        progress("timing the CPU baseline")
         out["cpu_baseline"] = cpu_baseline(cfg, th0[:1], nk, args.cpu_sample_modes, args.arm)
     print(json.dumps(out))
 

@@ -341,6 +341,243 @@ __device__ __forceinline__ T contract1(Ctx &cx, const uint32_t (&rd)[NS], T a, d
   return tot[0];
 }
 
+// ---------------------------------------------------------------------------
+// First-order contraction through background features.
+//
+// Every coefficient of row_visit depends on (a, theta) only through a few
+// wave-uniform features (aH, 1/aH, opac, cs2, pb43, the density weights
+// grhom Omega_c / a, ..., w, c_a^2, cs2_DE, a itself) and, for massive-nu rows,
+// the bin velocity v_q(a).  The derivative of sum_rows sum_fields mul * coef is
+// therefore sum_m Fbar_m dF_m + sum_lanes vbar dv_q, where Fbar_m (plain
+// doubles) are the partials of the row formulas below and dF_m, dv_q are the
+// features' tangents, computed once per contraction.
+enum : int {
+  FT_A = 0, FT_AH, FT_IAH, FT_OPAC, FT_CS2, FT_PB43, FT_RC, FT_RB, FT_RG, FT_RR, FT_RN, FT_RQ, FT_W, FT_CA2,
+  FT_CS2Q, FT_N
+};
+
+template <class T>
+__device__ __forceinline__ void features_t(Ctx &cx, T a, T (&F)[FT_N]) {
+  using P = ParT<T>;
+  Bg<T> b = background<T>(cx, a);
+  const P grhom = cv<T>(cx, F_GRHOM), grhog = cv<T>(cx, F_GRHOG), grhor = cv<T>(cx, F_GRHOR);
+  const P Omegam = cv<T>(cx, F_OMEGAM), Omegab = cv<T>(cx, F_OMEGAB), OmegaDE = cv<T>(cx, F_OMEGADE);
+  const P Neff = cv<T>(cx, F_NEFF), Nmnu = cv<T>(cx, F_NMNU);
+  const T a2 = a * a;
+  F[FT_A] = a;
+  F[FT_AH] = b.aH;
+  F[FT_IAH] = 1.0 / b.aH;
+  F[FT_OPAC] = b.opac;
+  F[FT_CS2] = b.cs2;
+  F[FT_PB43] = b.pb43;
+  F[FT_RC] = grhom * (Omegam - Omegab) / a;
+  F[FT_RB] = grhom * Omegab / a;
+  F[FT_RG] = grhog / a2;
+  F[FT_RR] = grhor * Neff / a2;
+  F[FT_RN] = grhor * Nmnu / a2;
+  F[FT_RQ] = grhom * OmegaDE * b.rhoQ * a2;
+  F[FT_W] = b.w;
+  F[FT_CA2] = b.ca2;
+  F[FT_CS2Q] = tconst<T>(0.0) + cv<T>(cx, F_CS2DE);
+}
+
+// partials of one row's sum_fields mul[f] coef_f with respect to the features
+// (accumulated into fb) and to its bin velocity v (returned); fv: feature values
+__device__ __forceinline__ double row_feature_adj(uint32_t d, const Ctx &cx, const double (&fv)[FT_N], double vq,
+                                                  const double (&mul)[11], double (&fb)[FT_N]) {
+  const double k = cx.k, k2 = k * k;
+  const uint32_t t = rd_type(d);
+  const double l = (double)rd_l(d);
+  const int qi = (int)rd_q(d);
+  const double a = fv[FT_A], aH = fv[FT_AH], iaH = fv[FT_IAH], opac = fv[FT_OPAC], pb43 = fv[FT_PB43];
+  const double m_sub = mul[C_SUB], m_diag = mul[C_DIAG], m_sup = mul[C_SUP], m_uh = mul[C_UH];
+  const double m_upi = mul[C_UPI], m_ch = mul[C_CH], m_ceta = mul[C_CETA], m_c1 = mul[C_C1];
+  double vb = 0.0;
+  switch (t) {
+    case R_ETA:
+      fb[FT_IAH] += m_ch * 2.0 * k2;
+      break;
+    case R_DC:
+      fb[FT_RC] += m_ch * iaH - m_c1 * a;
+      fb[FT_IAH] += m_ch * fv[FT_RC];
+      fb[FT_A] += -m_c1 * fv[FT_RC];
+      break;
+    case R_DB:
+      fb[FT_RB] += m_ch * iaH - m_c1 * a;
+      fb[FT_IAH] += m_ch * fv[FT_RB];
+      fb[FT_A] += -m_c1 * fv[FT_RB];
+      break;
+    case R_TC:
+      fb[FT_AH] += -m_diag;
+      fb[FT_RC] += m_ceta * (0.5 / k2);
+      break;
+    case R_TB:
+      fb[FT_CS2] += m_sub * k2;
+      fb[FT_AH] += -m_diag;
+      fb[FT_PB43] += (m_sup - m_diag) * opac;
+      fb[FT_OPAC] += (m_sup - m_diag) * pb43;
+      fb[FT_RB] += m_ceta * (0.5 / k2);
+      break;
+    case R_DG:
+      fb[FT_RG] += m_ch * iaH - 2.0 * m_c1 * a;
+      fb[FT_IAH] += m_ch * fv[FT_RG];
+      fb[FT_A] += -2.0 * m_c1 * fv[FT_RG];
+      break;
+    case R_TG:
+      fb[FT_OPAC] += m_sub - m_diag;
+      fb[FT_RG] += m_ceta * (4.0 / 3.0) * (0.5 / k2);
+      break;
+    case R_F2:
+      fb[FT_OPAC] += -m_diag + 0.1 * m_upi;
+      break;
+    case R_G0:
+      fb[FT_OPAC] += -m_diag + 0.5 * m_upi;
+      break;
+    case R_G2:
+      fb[FT_OPAC] += -m_diag + 0.1 * m_upi;
+      break;
+    case R_FL:
+    case R_FLMAX:
+    case R_GL:
+    case R_GLMAX:
+      fb[FT_OPAC] += -m_diag;
+      break;
+    case R_DR:
+      fb[FT_RR] += m_ch * iaH - 2.0 * m_c1 * a;
+      fb[FT_IAH] += m_ch * fv[FT_RR];
+      fb[FT_A] += -2.0 * m_c1 * fv[FT_RR];
+      break;
+    case R_TR:
+      fb[FT_RR] += m_ceta * (4.0 / 3.0) * (0.5 / k2);
+      break;
+    case R_Q0: {
+      const double wq = cx.w[qi], RN = fv[FT_RN];
+      const double iv = 1.0 / vq;
+      vb += -k * m_sup;
+      // ch = RN wq / v * iaH
+      fb[FT_RN] += m_ch * wq * iv * iaH;
+      fb[FT_IAH] += m_ch * RN * wq * iv;
+      vb += -m_ch * RN * wq * iaH * iv * iv;
+      // c1 = -a (RN wq / v + RN wq v)
+      fb[FT_A] += -m_c1 * RN * wq * (iv + vq);
+      fb[FT_RN] += -m_c1 * a * wq * (iv + vq);
+      vb += -m_c1 * a * RN * wq * (1.0 - iv * iv);
+      break;
+    }
+    case R_Q1:
+      vb += k * (m_sub / 3.0 - 2.0 * m_sup / 3.0);
+      fb[FT_RN] += m_ceta * k * cx.w[qi] * (0.5 / k2);
+      break;
+    case R_Q2:
+      vb += k * (0.4 * m_sub - 0.6 * m_sup);
+      break;
+    case R_QL:
+      vb += k * (m_sub * (l / (2.0 * l + 1.0)) - m_sup * ((l + 1.0) / (2.0 * l + 1.0)));
+      break;
+    case R_QLMAX:
+      vb += k * m_sub;
+      break;
+    case R_DQ:
+      if (!cx.lambda) {
+        const double w = fv[FT_W], ca2 = fv[FT_CA2], cs2Q = fv[FT_CS2Q], RQ = fv[FT_RQ];
+        const double opw = 1.0 + w, g = aH * aH / k2;
+        // diag = -3 (cs2Q - w) aH
+        fb[FT_CS2Q] += -3.0 * aH * m_diag;
+        fb[FT_W] += 3.0 * aH * m_diag;
+        fb[FT_AH] += -3.0 * (cs2Q - w) * m_diag;
+        // sup = -opw - 9 opw (cs2Q - ca2) aH^2 / k2
+        fb[FT_W] += m_sup * (-1.0 - 9.0 * (cs2Q - ca2) * g);
+        fb[FT_CS2Q] += -9.0 * opw * g * m_sup;
+        fb[FT_CA2] += 9.0 * opw * g * m_sup;
+        fb[FT_AH] += -18.0 * opw * (cs2Q - ca2) * aH / k2 * m_sup;
+        // uh = -0.5 opw
+        fb[FT_W] += -0.5 * m_uh;
+        // ch = RQ iaH
+        fb[FT_RQ] += m_ch * iaH;
+        fb[FT_IAH] += m_ch * RQ;
+        // c1 = -a (RQ + 3 cs2Q RQ)
+        fb[FT_A] += -m_c1 * RQ * (1.0 + 3.0 * cs2Q);
+        fb[FT_RQ] += -m_c1 * a * (1.0 + 3.0 * cs2Q);
+        fb[FT_CS2Q] += -m_c1 * 3.0 * a * RQ;
+      }
+      break;
+    case R_TQ:
+      if (!cx.lambda) {
+        const double w = fv[FT_W], ca2 = fv[FT_CA2], cs2Q = fv[FT_CS2Q], RQ = fv[FT_RQ];
+        const double opw = 1.0 + w;
+        // sub = cs2Q / opw k2
+        fb[FT_CS2Q] += m_sub * k2 / opw;
+        fb[FT_W] += -m_sub * cs2Q * k2 / (opw * opw);
+        // diag = -(1 - 3 cs2Q) aH
+        fb[FT_CS2Q] += 3.0 * aH * m_diag;
+        fb[FT_AH] += -(1.0 - 3.0 * cs2Q) * m_diag;
+        // ceta = opw RQ 0.5 / k2
+        fb[FT_W] += m_ceta * RQ * (0.5 / k2);
+        fb[FT_RQ] += m_ceta * opw * (0.5 / k2);
+        // c1 = -a 9 (cs2Q - ca2) aH opw RQ / k2
+        const double c9 = 9.0 / k2;
+        fb[FT_A] += -m_c1 * c9 * (cs2Q - ca2) * aH * opw * RQ;
+        fb[FT_CS2Q] += -m_c1 * a * c9 * aH * opw * RQ;
+        fb[FT_CA2] += m_c1 * a * c9 * aH * opw * RQ;
+        fb[FT_AH] += -m_c1 * a * c9 * (cs2Q - ca2) * opw * RQ;
+        fb[FT_W] += -m_c1 * a * c9 * (cs2Q - ca2) * aH * RQ;
+        fb[FT_RQ] += -m_c1 * a * c9 * (cs2Q - ca2) * aH * opw;
+      }
+      break;
+    default:
+      break;
+  }
+  return vb;
+}
+
+// Derivative components of sum_rows w_row f_row(a, Y) (value component not
+// meaningful), first order, through the features; T = TD.
+template <int NS, class T>
+__device__ __forceinline__ T contract1f(Ctx &cx, const uint32_t (&rd)[NS], T a, const double (&Y)[NS],
+                                        const double (&w)[NS], const RowSums &rs) {
+  const int lane = lane_id();
+  T F[FT_N];
+  features_t<T>(cx, a, F);
+  double fv[FT_N];
+#pragma unroll
+  for (int m = 0; m < FT_N; ++m) fv[m] = vval(F[m]);
+  double fb[FT_N];
+#pragma unroll
+  for (int m = 0; m < FT_N; ++m) fb[m] = 0.0;
+  T accv = tconst<T>(0.0);
+  const double ydg = readlane(Y[0], L_DG), ytg = readlane(Y[0], L_TG);
+  const double wahp = readlane(w[0], L_AHP);
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    const double ym = from_prev(Y[s]), yp = from_next(Y[s]);
+    const double lp = (s == 0) ? (lane == L_DG ? ytg : (lane == L_TG ? ydg : 0.0)) : 0.0;
+    const double mul[11] = {w[s] * ym,    w[s] * Y[s],  w[s] * yp,    w[s] * rs.H,  w[s] * rs.E, w[s] * rs.Pi,
+                            Y[s] * rs.sh, Y[s] * rs.se, Y[s] * rs.sp, Y[s] * wahp, w[s] * lp};
+    const uint32_t ty = rd_type(rd[s]);
+    const bool qrow = ty >= R_Q0 && ty <= R_QLMAX;
+    T v = nu_v<T>(cx, a, qrow ? (int)rd_q(rd[s]) : 0);
+    const double vb = row_feature_adj(rd[s], cx, fv, vval(v), mul, fb);
+    if (qrow) accv = accv + v * vb;
+  }
+  if (lane == L_A) {  // f_a = aH a
+    fb[FT_AH] += w[0] * fv[FT_A];
+    fb[FT_A] += w[0] * fv[FT_AH];
+  }
+  constexpr int C = ncomp<T>::value;
+  double red[FT_N + C];
+#pragma unroll
+  for (int m = 0; m < FT_N; ++m) red[m] = fb[m];
+#pragma unroll
+  for (int i = 0; i < C; ++i) red[FT_N + i] = comp(accv, i);
+  wave_sum<FT_N + C>(red);
+  T out;
+#pragma unroll
+  for (int i = 0; i < C; ++i) comp(out, i) = red[FT_N + i];
+#pragma unroll
+  for (int m = 0; m < FT_N; ++m) out = out + F[m] * red[m];
+  return out;
+}
+
 // per-lane multiplier of the explicit tau dependence (hierarchy truncations):
 // d f_row / d tau = trunc_row / tau^2 * y_row with trunc_row = l + 1
 template <int NS>
@@ -453,7 +690,7 @@ __device__ __forceinline__ void rodas5_step_vjp(Mode<NS> &m, Ctx &cx, double t0,
     {
       const double ai = readlane(u[0], L_A);
       const RowSums rs = coef_pass<NS, true>(cx, m.rd, ai, iti, rb, u, ub);
-      TD g = contract1<NS, TD>(cx, m.rd, seed_td(ai, 1.0), iti, u, rb, rs);
+      TD g = contract1f<NS, TD>(cx, m.rd, seed_td(ai, 1.0), u, rb, rs);
       if (lane == L_A) ub[0] += g.d[0];
 #pragma unroll
       for (int j = 0; j < MAXDIR; ++j) th_acc[j] += g.d[j + 1];
@@ -473,7 +710,7 @@ __device__ __forceinline__ void rodas5_step_vjp(Mode<NS> &m, Ctx &cx, double t0,
       if (lane == L_A) wr[0] = 0.0;
       double unused[NS];
       const RowSums rs = coef_pass<NS, false>(cx, m.rd, a0, it0, wr, ki, unused);
-      TD g = contract1<NS, TD>(cx, m.rd, seed_td(a0, 1.0), it0, ki, wr, rs);
+      TD g = contract1f<NS, TD>(cx, m.rd, seed_td(a0, 1.0), ki, wr, rs);
       a0b += g.d[0];
 #pragma unroll
       for (int j = 0; j < MAXDIR; ++j) th_acc[j] += g.d[j + 1];

@@ -1,7 +1,5 @@
 // gfx950 kernels and C ABI of the gradient path.
 //
-//   deb_forward_tape_kernel  the forward solve of deb_forward_kernel that also
-//                            writes every accepted state to the checkpoint tape
 //   deb_adjoint_kernel       one wavefront per (cosmology, k) item: reads the
 //                            mode's states back last-to-first, reverses every
 //                            step, the output interpolation, the initial
@@ -20,16 +18,6 @@
 #include "deb_host.h"
 
 namespace deb {
-
-template <int NS>
-__global__ void __launch_bounds__(WAVE) deb_forward_tape_kernel(const Params<NS> P, const TapeView T) {
-  __shared__ double lds[WAVE * NS];
-  const int item = blockIdx.x;
-  if (item >= P.n_cosmo * P.nk) return;
-  const int ki = P.nk - 1 - item / P.n_cosmo;
-  const int ci = item % P.n_cosmo;
-  integrate_mode<NS, true>(P, ci, ki, lds, &T);
-}
 
 template <int NS>
 __global__ void __launch_bounds__(WAVE) deb_adjoint_kernel(const Params<NS> P, const AdjArgs A) {
@@ -81,7 +69,7 @@ __global__ void __launch_bounds__(WAVE) deb_rhs_vjp_kernel(const Params<NS> P, c
   const double tau = H.t0[item], it = 1.0 / tau;
   const double a = readlane(y[0], L_A);
   const RowSums rs = coef_pass<NS, true>(cx, rd, a, it, w, y, yb);
-  TD g = contract1<NS, TD>(cx, rd, seed_td(a, 1.0), it, y, w, rs);
+  TD g = contract1f<NS, TD>(cx, rd, seed_td(a, 1.0), y, w, rs);
   if (lane == L_A) yb[0] += g.d[0];
   trunc_weights<NS>(rd, tw);
   double pt = 0.0;
@@ -140,20 +128,6 @@ __global__ void __launch_bounds__(WAVE) deb_step_vjp_kernel(const Params<NS> P, 
 // ---------------------------------------------------------------------------
 // host side
 
-inline int tape_view(TapeView &tv, void *tape, size_t bytes, long long n_modes, int ns) {
-  const long long pool = tape_pool_off(n_modes);
-  const long long cb = tape_chunk_bytes(ns);
-  if (!tape || (long long)bytes < pool + cb) return fail(DEB_E_ARG, "tape smaller than deb_tape_bytes");
-  char *base = static_cast<char *>(tape);
-  tv.counter = reinterpret_cast<unsigned long long *>(base);
-  tv.capacity = ((long long)bytes - pool) / cb;
-  tv.last = reinterpret_cast<long long *>(base + tape_last_off());
-  tv.count = reinterpret_cast<int32_t *>(base + tape_count_off(n_modes));
-  tv.pool = base + pool;
-  tv.chunk_bytes = cb;
-  return 0;
-}
-
 // reference indices of the state entries read by the output conversion
 inline int output_inputs(const deb_config_t &c, int32_t *idx) {
   const int lg = c.lmaxg, lgp = c.lmaxgp, lr = c.lmaxr, nq = c.nqmax;
@@ -169,24 +143,6 @@ inline int output_inputs(const deb_config_t &c, int32_t *idx) {
   idx[m++] = n - 1;
   for (int i = m; i < WAVE; ++i) idx[i] = -1;
   return m;
-}
-
-template <int NS>
-int run_forward_tape(const deb_config_t *cfg, const Layout &L, int n_cosmo, const deb_cosmo_t *cosmo,
-                     const double *tables, int64_t stride, const int32_t *desc, const double *q, const double *w,
-                     int nk, const double *kmodes, int nout, const double *tau_out, double *y_out, int32_t *status,
-                     int32_t *nsteps, void *tape, size_t tape_bytes, hipStream_t st) {
-  Params<NS> P;
-  if (int r = fill_params<NS>(P, cfg, L, n_cosmo, cosmo, tables, stride, desc, q, w, nk, kmodes, nout, tau_out))
-    return r;
-  P.y_out = y_out;
-  P.status = status;
-  P.nsteps = nsteps;
-  TapeView tv;
-  if (int r = tape_view(tv, tape, tape_bytes, (long long)n_cosmo * nk, NS)) return r;
-  if (int r = hip_check(hipMemsetAsync(tape, 0, 256, st), "tape header reset")) return r;
-  hipLaunchKernelGGL(deb_forward_tape_kernel<NS>, dim3((unsigned)((int64_t)n_cosmo * nk)), dim3(WAVE), 0, st, P, tv);
-  return hip_check(hipGetLastError(), "taped forward kernel launch");
 }
 
 template <int NS>
@@ -244,20 +200,6 @@ int deb_tape_bytes(const deb_config_t *cfg, int n_cosmo, int nk, int64_t n_state
   const long long chunks = n_states / TAPE_CH + n_modes + 1;
   *bytes = (size_t)(tape_pool_off(n_modes) + chunks * tape_chunk_bytes(L.ns));
   return 0;
-}
-
-int deb_solve_forward_tape(const deb_config_t *cfg, int n_cosmo, const deb_cosmo_t *cosmo, const double *tables,
-                           int64_t table_stride, const int32_t *desc, const double *q, const double *w, int nk,
-                           const double *kmodes, int nout, const double *tau_out, double *y_out, int32_t *status,
-                           int32_t *nsteps, void *tape, size_t tape_bytes, void *stream) {
-  if (int r = check_common(cfg, n_cosmo, cosmo, tables, desc, q, w)) return r;
-  if (nk < 1 || nout < 1 || !kmodes || !tau_out || !y_out || !status) return fail(DEB_E_ARG, "invalid mode arrays");
-  if ((int64_t)n_cosmo * nk > 0x7fffffff) return fail(DEB_E_ARG, "too many modes");
-  Layout L;
-  if (int r = build_layout(*cfg, L)) return r;
-  hipStream_t st = (hipStream_t)stream;
-  DEB_DISPATCH(L.ns, (run_forward_tape<NS>(cfg, L, n_cosmo, cosmo, tables, table_stride, desc, q, w, nk, kmodes, nout,
-                                           tau_out, y_out, status, nsteps, tape, tape_bytes, st)))
 }
 
 int deb_solve_adjoint(const deb_config_t *cfg, int n_cosmo, const deb_cosmo_t *cosmo, const double *tables,

@@ -160,6 +160,20 @@ inline int hip_check(hipError_t e, const char *what) {
   return 0;
 }
 
+inline int tape_view(TapeView &tv, void *tape, size_t bytes, long long n_modes, int ns) {
+  const long long pool = tape_pool_off(n_modes);
+  const long long cb = tape_chunk_bytes(ns);
+  if (!tape || (long long)bytes < pool + cb) return fail(DEB_E_ARG, "tape smaller than deb_tape_bytes");
+  char *base = static_cast<char *>(tape);
+  tv.counter = reinterpret_cast<unsigned long long *>(base);
+  tv.capacity = ((long long)bytes - pool) / cb;
+  tv.last = reinterpret_cast<long long *>(base + tape_last_off());
+  tv.count = reinterpret_cast<int32_t *>(base + tape_count_off(n_modes));
+  tv.pool = base + pool;
+  tv.chunk_bytes = cb;
+  return 0;
+}
+
 // DEB_NS_MASK (bit NS) limits which slot counts are compiled (development
 // builds); the default compiles all of 1..6.
 #ifndef DEB_NS_MASK

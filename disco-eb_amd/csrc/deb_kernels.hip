@@ -5,6 +5,8 @@
 //                       determine_starting_time, adiabatic ICs and the
 //                       adaptive Rodas5 loop (diffrax PID semantics), and
 //                       writes the output fields at every tau_out.
+//   deb_forward_tape_kernel  the same solve, also writing every accepted state
+//                       to the checkpoint tape of the gradient path.
 //   deb_rhs_kernel      test hook: model_synchronous for given states.
 //   deb_step_kernel     test hook: one Rodas5Transformed step.
 #include <hip/hip_runtime.h>
@@ -22,6 +24,18 @@
 namespace deb {
 
 thread_local std::string g_err;
+
+// The forward solve that also writes every accepted state to the checkpoint
+// tape of the gradient path (deb_solve_forward_tape).
+template <int NS>
+__global__ void __launch_bounds__(WAVE) deb_forward_tape_kernel(const Params<NS> P, const TapeView T) {
+  __shared__ double lds[WAVE * NS];
+  const int item = blockIdx.x;
+  if (item >= P.n_cosmo * P.nk) return;
+  const int ki = P.nk - 1 - item / P.n_cosmo;
+  const int ci = item % P.n_cosmo;
+  integrate_mode<NS, true>(P, ci, ki, lds, &T);
+}
 
 // ---------------------------------------------------------------------------
 // test hooks
@@ -114,6 +128,24 @@ int run_forward(const deb_config_t *cfg, const Layout &L, int n_cosmo, const deb
 }
 
 template <int NS>
+int run_forward_tape(const deb_config_t *cfg, const Layout &L, int n_cosmo, const deb_cosmo_t *cosmo,
+                     const double *tables, int64_t stride, const int32_t *desc, const double *q, const double *w,
+                     int nk, const double *kmodes, int nout, const double *tau_out, double *y_out, int32_t *status,
+                     int32_t *nsteps, void *tape, size_t tape_bytes, hipStream_t st) {
+  Params<NS> P;
+  if (int r = fill_params<NS>(P, cfg, L, n_cosmo, cosmo, tables, stride, desc, q, w, nk, kmodes, nout, tau_out))
+    return r;
+  P.y_out = y_out;
+  P.status = status;
+  P.nsteps = nsteps;
+  TapeView tv;
+  if (int r = tape_view(tv, tape, tape_bytes, (long long)n_cosmo * nk, NS)) return r;
+  if (int r = hip_check(hipMemsetAsync(tape, 0, 256, st), "tape header reset")) return r;
+  hipLaunchKernelGGL(deb_forward_tape_kernel<NS>, dim3((unsigned)((int64_t)n_cosmo * nk)), dim3(WAVE), 0, st, P, tv);
+  return hip_check(hipGetLastError(), "taped forward kernel launch");
+}
+
+template <int NS>
 int run_hook(bool step, const deb_config_t *cfg, const Layout &L, const deb_cosmo_t *cosmo, const double *tables,
              int64_t stride, const int32_t *desc, const double *q, const double *w, int m, const int32_t *ci,
              const double *kk, const double *t0, const double *t1, const double *y0, double *y1, double *err,
@@ -167,6 +199,20 @@ int deb_solve_forward(const deb_config_t *cfg, int n_cosmo, const deb_cosmo_t *c
   hipStream_t st = (hipStream_t)stream;
   DEB_DISPATCH(L.ns, (run_forward<NS>(cfg, L, n_cosmo, cosmo, tables, table_stride, desc, q, w, nk, kmodes, nout,
                                       tau_out, y_out, status, nsteps, workspace, st)))
+}
+
+int deb_solve_forward_tape(const deb_config_t *cfg, int n_cosmo, const deb_cosmo_t *cosmo, const double *tables,
+                           int64_t table_stride, const int32_t *desc, const double *q, const double *w, int nk,
+                           const double *kmodes, int nout, const double *tau_out, double *y_out, int32_t *status,
+                           int32_t *nsteps, void *tape, size_t tape_bytes, void *stream) {
+  if (int r = check_common(cfg, n_cosmo, cosmo, tables, desc, q, w)) return r;
+  if (nk < 1 || nout < 1 || !kmodes || !tau_out || !y_out || !status) return fail(DEB_E_ARG, "invalid mode arrays");
+  if ((int64_t)n_cosmo * nk > 0x7fffffff) return fail(DEB_E_ARG, "too many modes");
+  Layout L;
+  if (int r = build_layout(*cfg, L)) return r;
+  hipStream_t st = (hipStream_t)stream;
+  DEB_DISPATCH(L.ns, (run_forward_tape<NS>(cfg, L, n_cosmo, cosmo, tables, table_stride, desc, q, w, nk, kmodes, nout,
+                                           tau_out, y_out, status, nsteps, tape, tape_bytes, st)))
 }
 
 int deb_rhs(const deb_config_t *cfg, const deb_cosmo_t *cosmo, const double *tables, int64_t table_stride,
