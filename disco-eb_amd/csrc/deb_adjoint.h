@@ -25,6 +25,11 @@ constexpr int MAXDIR = DEB_MAXDIR;
 using TD = dn<1 + MAXDIR>;          // (a or tau, theta_1..theta_MAXDIR)
 using TH = dn<1 + MAXDIR, dual>;    // the same, each component also differentiated along a
 
+// The row passes of the reverse step are real calls rather than inlined:
+// inlined into the step they leave one kernel body with thousands of spilled
+// registers; as calls each is register-allocated on its own.
+#define DEB_OUTLINE __device__ __noinline__
+
 // Rodas5 tableau in flattened form (constant memory, read with wave-uniform
 // indices): stage point u_i = y0 + sum_j kA[i][j] k_j, y1 = y0 + sum_j kA8[j] k_j,
 // stage time t0 + kc[i] h, kd[i] h dT added in stage i, kC[i][j] / h k_j.
@@ -270,7 +275,7 @@ struct RowSums {
 };
 
 template <int NS, bool TAPPLY>
-__device__ __forceinline__ RowSums coef_pass(Ctx &cx, const uint32_t (&rd)[NS], double a, double invtau,
+DEB_OUTLINE RowSums coef_pass(Ctx &cx, const uint32_t (&rd)[NS], double a, double invtau,
                                              const double (&w)[NS], const double (&Y)[NS], double (&out)[NS]) {
   const int lane = lane_id();
   Bg<double> b = background<double>(cx, a);
@@ -308,37 +313,6 @@ __device__ __forceinline__ RowSums coef_pass(Ctx &cx, const uint32_t (&rd)[NS], 
     for (int s = 0; s < NS; ++s) out[s] += cc[0][s] * rs.sh + cc[1][s] * rs.se + cc[2][s] * rs.sp + cc[3][s] * wahp;
   }
   return rs;
-}
-
-// First-order contraction: the derivative components of the returned T equal
-// those of sum_rows w_row f_row(a, Y) (the value component does not).  The
-// functionals enter through their double values (rs), which is exact for
-// first derivatives: d(u H) = du H + u dH and sum_rows w u dH = s_h dH.
-// Coefficients are streamed through row_visit, so no row is held whole.
-template <int NS, class T>
-__device__ __forceinline__ T contract1(Ctx &cx, const uint32_t (&rd)[NS], T a, double invtau, const double (&Y)[NS],
-                                       const double (&w)[NS], const RowSums &rs) {
-  const int lane = lane_id();
-  Bg<T> b = background<T>(cx, a);
-  T acc = tconst<T>(0.0);
-  const double ydg = readlane(Y[0], L_DG), ytg = readlane(Y[0], L_TG);
-  const double wahp = readlane(w[0], L_AHP);
-#pragma unroll
-  for (int s = 0; s < NS; ++s) {
-    const double ym = from_prev(Y[s]), yp = from_next(Y[s]);
-    const double lp = (s == 0) ? (lane == L_DG ? ytg : (lane == L_TG ? ydg : 0.0)) : 0.0;
-    // multiplier of each coefficient field in the contraction
-    const double mul[11] = {w[s] * ym,    w[s] * Y[s],    w[s] * yp,       w[s] * rs.H, w[s] * rs.E,
-                            w[s] * rs.Pi, Y[s] * rs.sh,   Y[s] * rs.se,    Y[s] * rs.sp, Y[s] * wahp,
-                            w[s] * lp};
-    row_visit<T>(rd[s], b, cx, invtau, [&](auto f, const auto &v) __attribute__((always_inline)) {
-      acc = acc + v * mul[decltype(f)::value];
-    });
-  }
-  if (lane == L_A) acc = acc + b.aH * b.a * w[0];
-  T tot[1] = {acc};
-  t_wave_sum<T, 1>(tot);
-  return tot[0];
 }
 
 // ---------------------------------------------------------------------------
@@ -531,9 +505,12 @@ __device__ __forceinline__ double row_feature_adj(uint32_t d, const Ctx &cx, con
 }
 
 // Derivative components of sum_rows w_row f_row(a, Y) (value component not
-// meaningful), first order, through the features; T = TD.
+// meaningful), first order, through the features; T = TD.  The functionals
+// (h', eta', Pi, f_1) enter through their double values in rs, which is exact
+// at first order: d(u H) = du H + u dH with dH = sum_rows dc_h Y, so
+// sum_rows w u dH = s_h sum_rows dc_h Y (multiplier Y s_h of the c_h field).
 template <int NS, class T>
-__device__ __forceinline__ T contract1f(Ctx &cx, const uint32_t (&rd)[NS], T a, const double (&Y)[NS],
+DEB_OUTLINE T contract1f(Ctx &cx, const uint32_t (&rd)[NS], T a, const double (&Y)[NS],
                                         const double (&w)[NS], const RowSums &rs) {
   const int lane = lane_id();
   T F[FT_N];

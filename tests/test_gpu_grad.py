@@ -26,6 +26,13 @@ def _cfg(**kw):
     return packing.make_config(**{**CFG, **kw})
 
 
+def _layout(name):
+    """The small test layout or the production one (lmax 32, lmaxnu 17, nq 5)."""
+    if name == "small":
+        return _cfg()
+    return packing.make_config(lmaxg=32, lmaxgp=32, lmaxr=32, lmaxnu=17, nqmax=5)
+
+
 def _setup(g, cfg, gpu):
     inp = DeviceInputs(g["param"], cfg, gpu)
     tan = gradient.Tangents(g["dcos"], g["dtab"], g["dtau"], gpu)
@@ -76,47 +83,85 @@ def _states(prob, kk, advance, seed):
     return np.array(ts), np.array(ys)
 
 
-def test_rhs_vjp_dot_product(grad_inputs, gpu):
+def _late(prob, param, ys, a_late):
+    """The same states moved to late scale factors (y[0] = a) and their conformal times."""
+    ys = ys.copy()
+    ys[:, 0] = a_late
+    return packing.tau_of_aexp(param, a_late).real[0], ys
+
+
+@pytest.mark.parametrize("era", ["early", "late"])
+@pytest.mark.parametrize("cfgname", ["small", "lmax32-nq5"])
+def test_rhs_vjp_dot_product(grad_inputs, gpu, cfgname, era):
     g = grad_inputs
-    cfg = _cfg()
+    cfg = _layout(cfgname)
     prob = oracle.Problem(g["param"], cfg)
     inp, tan = _setup(g, cfg, gpu)
     kk = np.array([1e-3, 0.03, 0.4, 2.0])
     taus, ys = _states(prob, kk, 4, 1)
+    if era == "late":  # dark energy and massive-nu rows carry weight only here
+        taus, ys = _late(prob, g["param"], ys, np.array([0.2, 0.5, 0.8, 1.0]))
     rng = np.random.default_rng(2)
     w = rng.standard_normal(ys.shape)
     ybar, taub, _, grad = _vjp("rhs", cfg, inp, tan, kk, taus, ys, w)
+    zc, zt = np.zeros_like(g["dcos"][0, 0]), np.zeros_like(g["dtab"][0, 0])
     for i, k in enumerate(kk):
         dy = rng.standard_normal(ys.shape[1]) * np.abs(ys[i])
         dtau = 0.1 * taus[i]
-        for j in range(tan.ndir):
-            f, df = prob.rhs_tangent(k, taus[i], dtau, ys[i], dy, g["dcos"][0, j], g["dtab"][0, j])
+        # state and time part, parameters fixed
+        _, df = prob.rhs_tangent(k, taus[i], dtau, ys[i], dy, zc, zt)
+        lhs, parts = w[i] @ df, np.array([ybar[i] @ dy, taub[i] * dtau])
+        scale = np.abs(w[i] * df).sum() + np.abs(ybar[i] * dy).sum() + abs(parts[1])
+        assert abs(lhs - parts.sum()) <= 1e-11 * scale, (k, lhs, parts)
+        # parameter part alone; a direction whose derivative nearly cancels (h at fixed omega_c, omega_b
+        # early on) is measured against the largest direction's magnitude
+        dfs = [prob.rhs_tangent(k, taus[i], 0.0, ys[i], np.zeros_like(dy), g["dcos"][0, j], g["dtab"][0, j])[1]
+               for j in range(tan.ndir)]
+        mags = np.array([np.abs(w[i] * df).sum() for df in dfs])
+        for j, df in enumerate(dfs):
             lhs = w[i] @ df
-            parts = np.array([ybar[i] @ dy, taub[i] * dtau, grad[i, j]])
-            scale = np.abs(w[i] * df).sum() + np.abs(ybar[i] * dy).sum() + abs(parts[1]) + abs(parts[2])
-            assert abs(lhs - parts.sum()) <= 1e-11 * scale, (k, j, lhs, parts)
+            assert abs(lhs - grad[i, j]) <= 1e-11 * max(mags[j], 1e-3 * mags.max()), (k, j, lhs, grad[i, j])
 
 
-def test_step_vjp_dot_product(grad_inputs, gpu):
+@pytest.mark.parametrize("era", ["early", "late"])
+@pytest.mark.parametrize("cfgname", ["small", "lmax32-nq5"])
+def test_step_vjp_dot_product(grad_inputs, gpu, cfgname, era):
     g = grad_inputs
-    cfg = _cfg()
+    cfg = _layout(cfgname)
     prob = oracle.Problem(g["param"], cfg)
     inp, tan = _setup(g, cfg, gpu)
     kk = np.array([2e-3, 0.05, 0.3])
     t0, ys = _states(prob, kk, 6, 3)
     t1 = 1.4 * t0
+    if era == "late":
+        t0, ys = _late(prob, g["param"], ys, np.array([0.3, 0.6, 0.9]))
+        t1 = 1.05 * t0
     rng = np.random.default_rng(4)
     w = rng.standard_normal(ys.shape)
     y0b, t0b, t1b, grad = _vjp("step", cfg, inp, tan, kk, t0, ys, w, t1)
+    zc, zt = np.zeros_like(g["dcos"][0, 0]), np.zeros_like(g["dtab"][0, 0])
+    # the production layout at k = 0.3 agrees at the 1e-9 level only (the oracle's dense LU against the
+    # structured solve of a stiffer system); the small layout at 1e-10
+    tol = 1e-9 if cfgname == "small" else 1e-8
     for i, k in enumerate(kk):
         dy0 = rng.standard_normal(ys.shape[1]) * np.abs(ys[i])
-        dt0, dt1 = 0.2 * t0[i], 0.3 * t0[i]
-        for j in range(tan.ndir):
-            y1, dy1 = prob.step_tangent(k, t0[i], dt0, t1[i], dt1, ys[i], dy0, g["dcos"][0, j], g["dtab"][0, j])
+        dt0, dt1 = 0.2 * (t1[i] - t0[i]), 0.3 * (t1[i] - t0[i])
+        _, dy1 = prob.step_tangent(k, t0[i], dt0, t1[i], dt1, ys[i], dy0, zc, zt)
+        lhs = w[i] @ dy1
+        parts = np.array([y0b[i] @ dy0, t0b[i] * dt0, t1b[i] * dt1])
+        scale = np.abs(w[i] * dy1).sum() + np.abs(y0b[i] * dy0).sum() + np.abs(parts[1:]).sum()
+        assert abs(lhs - parts.sum()) <= tol * scale, (k, lhs, parts)
+        dys = [prob.step_tangent(k, t0[i], 0.0, t1[i], 0.0, ys[i], np.zeros_like(dy0), g["dcos"][0, j],
+                                 g["dtab"][0, j])[1] for j in range(tan.ndir)]
+        mags = np.array([np.abs(w[i] * d).sum() for d in dys])
+        for j, dy1 in enumerate(dys):
             lhs = w[i] @ dy1
-            parts = np.array([y0b[i] @ dy0, t0b[i] * dt0, t1b[i] * dt1, grad[i, j]])
-            scale = np.abs(w[i] * dy1).sum() + np.abs(y0b[i] * dy0).sum() + np.abs(parts[1:]).sum()
-            assert abs(lhs - parts.sum()) <= 1e-9 * scale, (k, j, lhs, parts)
+            assert abs(lhs - grad[i, j]) <= tol * max(mags[j], 1e-3 * mags.max()), (k, j, lhs, grad[i, j])
+        # every state component, unscaled (entries with tiny values, e.g. the truncation multipoles, count fully)
+        du = rng.standard_normal(ys.shape[1])
+        _, dy1 = prob.step_tangent(k, t0[i], 0.0, t1[i], 0.0, ys[i], du, zc, zt)
+        lhs = w[i] @ dy1
+        assert abs(lhs - y0b[i] @ du) <= tol * (np.abs(w[i] * dy1).sum() + np.abs(y0b[i] * du).sum()), (k, lhs)
 
 
 def _tape_times(cfg, tape, n_modes):
