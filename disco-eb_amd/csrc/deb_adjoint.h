@@ -27,7 +27,11 @@ using TH = dn<1 + MAXDIR, dual>;    // the same, each component also differentia
 
 // The row passes of the reverse step are real calls rather than inlined:
 // inlined into the step they leave one kernel body with thousands of spilled
-// registers; as calls each is register-allocated on its own.
+// registers; as calls each is register-allocated on its own.  An outlined
+// function must stay small enough to need no long branches: the compiler's
+// long-branch sequences in a non-kernel function were seen to use the return
+// address registers s[30:31] without saving them (a wild return).  The
+// device-assembly test of the build checks this for every outlined function.
 #define DEB_OUTLINE __device__ __noinline__
 
 // Rodas5 tableau in flattened form (constant memory, read with wave-uniform
@@ -231,7 +235,7 @@ __device__ __forceinline__ T contract(Ctx &cx, const uint32_t (&rd)[NS], T a, do
 // out = A(a)^T w for the y-linear part of f (rows other than a; the a-column
 // and the a-row are handled by contract).  T = dual gives (A^T w, A'(a)^T w).
 template <int NS, class T>
-__device__ __forceinline__ void tapply(Ctx &cx, const uint32_t (&rd)[NS], T a, double invtau, const double (&w)[NS],
+DEB_OUTLINE void tapply(Ctx &cx, const uint32_t (&rd)[NS], T a, double invtau, const double (&w)[NS],
                                        T (&out)[NS]) {
   const int lane = lane_id();
   Bg<T> b = background<T>(cx, a);
@@ -320,14 +324,15 @@ DEB_OUTLINE RowSums coef_pass(Ctx &cx, const uint32_t (&rd)[NS], double a, doubl
 //
 // Every coefficient of row_visit depends on (a, theta) only through a few
 // wave-uniform features (aH, 1/aH, opac, cs2, pb43, the density weights
-// grhom Omega_c / a, ..., w, c_a^2, cs2_DE, a itself) and, for massive-nu rows,
-// the bin velocity v_q(a).  The derivative of sum_rows sum_fields mul * coef is
-// therefore sum_m Fbar_m dF_m + sum_lanes vbar dv_q, where Fbar_m (plain
-// doubles) are the partials of the row formulas below and dF_m, dv_q are the
-// features' tangents, computed once per contraction.
+// grhom Omega_c / a, ..., w, c_a^2, cs2_DE, a itself, a m_nu) and, for
+// massive-nu rows, the bin velocity v_q = (1 + (a amnu / q)^2)^(-1/2), whose
+// partials are folded into those of a and amnu.  The derivative of
+// sum_rows sum_fields mul * coef is therefore sum_m Fbar_m dF_m, where Fbar_m
+// (plain doubles) are the partials of the row formulas below and dF_m the
+// features' tangents.
 enum : int {
   FT_A = 0, FT_AH, FT_IAH, FT_OPAC, FT_CS2, FT_PB43, FT_RC, FT_RB, FT_RG, FT_RR, FT_RN, FT_RQ, FT_W, FT_CA2,
-  FT_CS2Q, FT_N
+  FT_CS2Q, FT_AMNU, FT_N
 };
 
 template <class T>
@@ -353,6 +358,7 @@ __device__ __forceinline__ void features_t(Ctx &cx, T a, T (&F)[FT_N]) {
   F[FT_W] = b.w;
   F[FT_CA2] = b.ca2;
   F[FT_CS2Q] = tconst<T>(0.0) + cv<T>(cx, F_CS2DE);
+  F[FT_AMNU] = tconst<T>(0.0) + cv<T>(cx, F_AMNU);
 }
 
 // partials of one row's sum_fields mul[f] coef_f with respect to the features
@@ -504,24 +510,20 @@ __device__ __forceinline__ double row_feature_adj(uint32_t d, const Ctx &cx, con
   return vb;
 }
 
-// Derivative components of sum_rows w_row f_row(a, Y) (value component not
-// meaningful), first order, through the features; T = TD.  The functionals
-// (h', eta', Pi, f_1) enter through their double values in rs, which is exact
-// at first order: d(u H) = du H + u dH with dH = sum_rows dc_h Y, so
-// sum_rows w u dH = s_h sum_rows dc_h Y (multiplier Y s_h of the c_h field).
-template <int NS, class T>
-DEB_OUTLINE T contract1f(Ctx &cx, const uint32_t (&rd)[NS], T a, const double (&Y)[NS],
-                                        const double (&w)[NS], const RowSums &rs) {
+// First-order contraction of sum_rows w_row f_row(a, Y) along (a, theta):
+//   feature_partials per lane, the partials of its rows' sum_fields mul * coef
+//                    with respect to the features, accumulated into fb (plain
+//                    doubles, no cross-lane sums);
+//   lane_dot         after fb is summed over the wave, sum_m dF_m fb_m for the
+//                    lane's own tangent direction (lane_ctx).
+// The functionals (h', eta', Pi, f_1) enter through their double values in rs,
+// which is exact at first order: d(u H) = du H + u dH with dH = sum_rows dc_h Y,
+// so sum_rows w u dH = s_h sum_rows dc_h Y (multiplier Y s_h of the c_h field).
+template <int NS>
+DEB_OUTLINE void feature_partials(const Ctx &cx, const uint32_t (&rd)[NS], const double (&fv)[FT_N],
+                                  const double (&Y)[NS], const double (&w)[NS], const RowSums &rs,
+                                  double (&fb)[FT_N]) {
   const int lane = lane_id();
-  T F[FT_N];
-  features_t<T>(cx, a, F);
-  double fv[FT_N];
-#pragma unroll
-  for (int m = 0; m < FT_N; ++m) fv[m] = vval(F[m]);
-  double fb[FT_N];
-#pragma unroll
-  for (int m = 0; m < FT_N; ++m) fb[m] = 0.0;
-  T accv = tconst<T>(0.0);
   const double ydg = readlane(Y[0], L_DG), ytg = readlane(Y[0], L_TG);
   const double wahp = readlane(w[0], L_AHP);
 #pragma unroll
@@ -532,27 +534,89 @@ DEB_OUTLINE T contract1f(Ctx &cx, const uint32_t (&rd)[NS], T a, const double (&
                             Y[s] * rs.sh, Y[s] * rs.se, Y[s] * rs.sp, Y[s] * wahp, w[s] * lp};
     const uint32_t ty = rd_type(rd[s]);
     const bool qrow = ty >= R_Q0 && ty <= R_QLMAX;
-    T v = nu_v<T>(cx, a, qrow ? (int)rd_q(rd[s]) : 0);
-    const double vb = row_feature_adj(rd[s], cx, fv, vval(v), mul, fb);
-    if (qrow) accv = accv + v * vb;
+    // v_q(x), x = a amnu / q: dv/dx = -x v^3
+    const double iq = qrow ? 1.0 / cx.q[rd_q(rd[s])] : 0.0;
+    const double x = fv[FT_A] * fv[FT_AMNU] * iq;
+    const double v = 1.0 / sqrt(1.0 + x * x);
+    const double vb = row_feature_adj(rd[s], cx, fv, v, mul, fb);
+    if (qrow) {
+      const double dvdx = -x * v * v * v;
+      fb[FT_A] += vb * dvdx * fv[FT_AMNU] * iq;
+      fb[FT_AMNU] += vb * dvdx * fv[FT_A] * iq;
+    }
   }
   if (lane == L_A) {  // f_a = aH a
     fb[FT_AH] += w[0] * fv[FT_A];
     fb[FT_A] += w[0] * fv[FT_AH];
   }
-  constexpr int C = ncomp<T>::value;
-  double red[FT_N + C];
+}
+
+// One tangent direction per lane: lane & 7 == 0 carries d/da (component 0 of
+// TL, seeded by seed_tl), lane & 7 == c (1 <= c <= ndir) the input tangent
+// c - 1 (component 1, loaded through lane_ctx); TL arithmetic then costs
+// three doubles per value instead of 2 + MAXDIR.  Only wave-uniform
+// quantities may be computed in TL (a lane sum would mix directions).
+using TL = dn<2>;
+
+__device__ __forceinline__ Ctx lane_ctx(const Ctx &cx) {
+  Ctx cl = cx;
+  const int c = lane_id() & 7;
+  const bool on = c >= 1 && c <= cx.ndir;
+  cl.ndir = on ? 1 : 0;
+  cl.dcos = cx.dcos + (on ? (c - 1) * COSMO_DOUBLES : 0);
+  cl.dtab = cx.dtab + (on ? (c - 1) * cx.stride : 0);
+  return cl;
+}
+
+__device__ __forceinline__ TL seed_tl(double a) {
+  TL r;
+  r = a;
+  r.d[0] = (lane_id() & 7) == 0 ? 1.0 : 0.0;
+  return r;
+}
+
+// the lane's direction of sum_m F_m fbs_m (fbs wave-uniform)
+__device__ __forceinline__ double lane_dot(const TL (&F)[FT_N], const double (&fbs)[FT_N]) {
+  double g = 0.0;
 #pragma unroll
-  for (int m = 0; m < FT_N; ++m) red[m] = fb[m];
+  for (int m = 0; m < FT_N; ++m) g += (F[m].d[0] + F[m].d[1]) * fbs[m];
+  return g;
+}
+
+// feature values at a (plain doubles)
+DEB_OUTLINE void feature_values(Ctx &cx, double a, double (&fv)[FT_N]) { features_t<double>(cx, a, fv); }
+
+// the lane's direction of sum_m F_m(a) fbs_m (fbs wave-uniform): d/da on
+// lanes with lane & 7 == 0, d/dtheta_{c-1} on lanes with lane & 7 == c
+DEB_OUTLINE double lane_contract(const Ctx &cx, double a, const double (&fbs)[FT_N]) {
+  Ctx cl = lane_ctx(cx);
+  TL F[FT_N];
+  features_t<TL>(cl, seed_tl(a), F);
+  return lane_dot(F, fbs);
+}
+
+// the lane-direction result of every direction: out[j] = d/dtheta_j
+__device__ __forceinline__ void lane_gather(double g, int ndir, double (&out)[MAXDIR]) {
 #pragma unroll
-  for (int i = 0; i < C; ++i) red[FT_N + i] = comp(accv, i);
-  wave_sum<FT_N + C>(red);
-  T out;
+  for (int j = 0; j < MAXDIR; ++j) out[j] = j < ndir ? readlane(g, j + 1) : 0.0;
+}
+
+// row sums of the step Jacobian J(t0, y0) kept by the factorisation
+template <int NS>
+__device__ __forceinline__ RowSums jac_sums(const Mode<NS> &m, const double (&Y)[NS], const double (&w)[NS]) {
+  double part[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-  for (int i = 0; i < C; ++i) comp(out, i) = red[FT_N + i];
-#pragma unroll
-  for (int m = 0; m < FT_N; ++m) out = out + F[m] * red[m];
-  return out;
+  for (int s = 0; s < NS; ++s) {
+    part[0] += m.Cf[0][s] * Y[s];
+    part[1] += m.Cf[1][s] * Y[s];
+    part[2] += m.Cf[2][s] * Y[s];
+    part[3] += m.c1[s] * Y[s];
+    part[4] += m.U[0][s] * w[s];
+    part[5] += m.U[1][s] * w[s];
+    part[6] += m.U[2][s] * w[s];
+  }
+  wave_sum<7>(part);
+  return RowSums{part[0], part[1], part[2], part[3], part[4], part[5], part[6]};
 }
 
 // per-lane multiplier of the explicit tau dependence (hierarchy truncations):
@@ -574,25 +638,32 @@ __device__ __forceinline__ TD seed_td(double v, double d0) {
   return r;
 }
 
+// the forward step from (t0, y0) again: factors into m, stage vectors into Kl
+// (inlined: as a call it is large enough to need long branches, which must
+// stay out of non-kernel functions)
+template <int NS>
+__device__ __forceinline__ void recompute_step(Mode<NS> &m, Ctx &cx, double t0, double h, const double (&y0)[NS],
+                                               double *Kl) {
+  const int lane = lane_id();
+  double y1[NS], err[NS];
+  rodas5_step_s<NS>(m, cx, t0, h, y0, y1, err, [&](int i, const double (&k)[NS]) {
+#pragma unroll
+    for (int s = 0; s < NS; ++s) Kl[(i * NS + s) * WAVE + lane] = k[s];
+  });
+}
+
 // ---------------------------------------------------------------------------
 // reverse of one Rodas5 step.  In: ybar = dL/dy1.  Out: ybar = dL/dy0,
-// tbar += dL/dt0 (h fixed), hbar += dL/dh, thb[j] += dL/dtheta_j.
+// tbar += dL/dt0 (h fixed), hbar += dL/dh, thb[j] += dL/dtheta_j (wave-uniform).
 // Kl / KBl: per-wave LDS scratch of 8 and 7 stage vectors.
 template <int NS>
-__device__ __forceinline__ void rodas5_step_vjp(Mode<NS> &m, Ctx &cx, double t0, double h, const double (&y0)[NS],
-                                                double (&ybar)[NS], double (&thb)[MAXDIR], double &tbar, double &hbar,
-                                                double *Kl, double *KBl) {
+__device__ __forceinline__ void rodas5_step_vjp(Mode<NS> &m, Ctx &cx, double t0, double h,
+                                                const double (&y0)[NS], double (&ybar)[NS], double (&thb)[MAXDIR],
+                                                double &tbar, double &hbar, double *Kl, double *KBl) {
   const int lane = lane_id();
   auto Kp = [&](int i, int s) -> double & { return Kl[(i * NS + s) * WAVE + lane]; };
   auto KBp = [&](int i, int s) -> double & { return KBl[(i * NS + s) * WAVE + lane]; };
-  // forward recompute, keeping every stage vector
-  {
-    double y1[NS], err[NS];
-    rodas5_step_s<NS>(m, cx, t0, h, y0, y1, err, [&](int i, const double (&k)[NS]) {
-#pragma unroll
-      for (int s = 0; s < NS; ++s) Kp(i, s) = k[s];
-    });
-  }
+  recompute_step<NS>(m, cx, t0, h, y0, Kl);
   double tw[NS];
   trunc_weights<NS>(m.rd, tw);
   const double it0 = 1.0 / t0;
@@ -618,6 +689,12 @@ __device__ __forceinline__ void rodas5_step_vjp(Mode<NS> &m, Ctx &cx, double t0,
 #pragma unroll
   for (int j = 0; j < MAXDIR; ++j) th_acc[j] = 0.0;
   double a0b = 0.0, t0b = 0.0, hb = 0.0;
+  // J(t0, y0) k_i terms of all stages share the features at a0: their
+  // partials are summed over the stages and contracted once after the loop
+  double fv0[FT_N], fbJ[FT_N];
+  feature_values(cx, a0, fv0);
+#pragma unroll
+  for (int q = 0; q < FT_N; ++q) fbJ[q] = 0.0;
 #pragma nounroll
   for (int i = 7; i >= 0; --i) {
     double rb[NS], ki[NS];
@@ -667,10 +744,19 @@ __device__ __forceinline__ void rodas5_step_vjp(Mode<NS> &m, Ctx &cx, double t0,
     {
       const double ai = readlane(u[0], L_A);
       const RowSums rs = coef_pass<NS, true>(cx, m.rd, ai, iti, rb, u, ub);
-      TD g = contract1f<NS, TD>(cx, m.rd, seed_td(ai, 1.0), u, rb, rs);
-      if (lane == L_A) ub[0] += g.d[0];
+      double fv[FT_N], fb[FT_N];
+      feature_values(cx, ai, fv);
 #pragma unroll
-      for (int j = 0; j < MAXDIR; ++j) th_acc[j] += g.d[j + 1];
+      for (int q = 0; q < FT_N; ++q) fb[q] = 0.0;
+      feature_partials<NS>(cx, m.rd, fv, u, rb, rs, fb);
+      wave_sum<FT_N>(fb);
+      const double g = lane_contract(cx, ai, fb);
+      const double ga = readlane(g, 0);
+      if (lane == L_A) ub[0] += ga;
+      double tl[MAXDIR];
+      lane_gather(g, cx.ndir, tl);
+#pragma unroll
+      for (int j = 0; j < MAXDIR; ++j) th_acc[j] += tl[j];
     }
 #pragma unroll
     for (int s = 0; s < NS; ++s) y0b[s] += ub[s];
@@ -685,16 +771,22 @@ __device__ __forceinline__ void rodas5_step_vjp(Mode<NS> &m, Ctx &cx, double t0,
 #pragma unroll
       for (int s = 0; s < NS; ++s) wr[s] = rb[s];
       if (lane == L_A) wr[0] = 0.0;
-      double unused[NS];
-      const RowSums rs = coef_pass<NS, false>(cx, m.rd, a0, it0, wr, ki, unused);
-      TD g = contract1f<NS, TD>(cx, m.rd, seed_td(a0, 1.0), ki, wr, rs);
-      a0b += g.d[0];
-#pragma unroll
-      for (int j = 0; j < MAXDIR; ++j) th_acc[j] += g.d[j + 1];
+      const RowSums rs = jac_sums<NS>(m, ki, wr);
+      feature_partials<NS>(cx, m.rd, fv0, ki, wr, rs, fbJ);
       const double kia = readlane(ki[0], L_A);
 #pragma unroll
       for (int s = 0; s < NS; ++s) rho[s] += kia * rb[s];
     }
+  }
+  // the J(t0, y0) k_i terms, contracted with the feature tangents at a0
+  {
+    wave_sum<FT_N>(fbJ);
+    const double g = lane_contract(cx, a0, fbJ);
+    a0b += readlane(g, 0);
+    double tl[MAXDIR];
+    lane_gather(g, cx.ndir, tl);
+#pragma unroll
+    for (int j = 0; j < MAXDIR; ++j) th_acc[j] += tl[j];
   }
   // a-column of J(t0, y0): rho^T df/da, differentiated along y0, a0 and theta
   {

@@ -69,8 +69,17 @@ __global__ void __launch_bounds__(WAVE) deb_rhs_vjp_kernel(const Params<NS> P, c
   const double tau = H.t0[item], it = 1.0 / tau;
   const double a = readlane(y[0], L_A);
   const RowSums rs = coef_pass<NS, true>(cx, rd, a, it, w, y, yb);
-  TD g = contract1f<NS, TD>(cx, rd, seed_td(a, 1.0), y, w, rs);
-  if (lane == L_A) yb[0] += g.d[0];
+  double fv[FT_N], fb[FT_N];
+  feature_values(cx, a, fv);
+#pragma unroll
+  for (int q = 0; q < FT_N; ++q) fb[q] = 0.0;
+  feature_partials<NS>(cx, rd, fv, y, w, rs, fb);
+  wave_sum<FT_N>(fb);
+  const double gl = lane_contract(cx, a, fb);
+  double gth[MAXDIR];
+  lane_gather(gl, H.ndir, gth);
+  const double ga = readlane(gl, 0);
+  if (lane == L_A) yb[0] += ga;
   trunc_weights<NS>(rd, tw);
   double pt = 0.0;
 #pragma unroll
@@ -83,7 +92,7 @@ __global__ void __launch_bounds__(WAVE) deb_rhs_vjp_kernel(const Params<NS> P, c
   if (lane == 0) H.t0bar[item] = pt;
 #pragma unroll
   for (int j = 0; j < MAXDIR; ++j)
-    if (lane == j && j < H.ndir) H.grad[(long long)item * H.ndir + j] = g.d[j + 1];
+    if (lane == j && j < H.ndir) H.grad[(long long)item * H.ndir + j] = gth[j];
 }
 
 template <int NS>

@@ -24,6 +24,7 @@ struct Mode {
   double Z[3][NS];   // W_B^{-1} [u_h, u_eta, u_pi]
   double Cf[3][NS];  // c_h, c_eta, c_pi
   double c1[NS];
+  double U[3][NS];   // u_h, u_eta, u_pi (where h', eta', Pi enter each row)
   double jcol[NS];   // df/da at the step start
   // uniform
   double gh;         // gamma * h
@@ -171,6 +172,9 @@ __device__ __forceinline__ void factor(Mode<NS> &m, Ctx &cx, double tau0, const 
     m.Cf[1][s] = co[s].ceta.v;
     m.Cf[2][s] = co[s].cpi.v;
     m.c1[s] = co[s].c1.v;
+    m.U[0][s] = co[s].uh.v;
+    m.U[1][s] = co[s].ueta.v;
+    m.U[2][s] = co[s].upi.v;
   }
   // leaf: delta_gamma (lane L_DG) couples only to theta_gamma (lane L_TG)
   const double w_dg_tg = -readlane(co[0].leaf.v, L_DG);
