@@ -270,6 +270,53 @@ DEB_OUTLINE void tapply(Ctx &cx, const uint32_t (&rd)[NS], T a, double invtau, c
   for (int s = 0; s < NS; ++s) out[s] = out[s] + cc[0][s] * su[0] + cc[1][s] * su[1] + cc[2][s] * su[2] + cc[3][s] * wahp;
 }
 
+// ---------------------------------------------------------------------------
+// First-order contraction through background features.
+//
+// Every coefficient of row_visit depends on (a, theta) only through a few
+// wave-uniform features (aH, 1/aH, opac, cs2, pb43, the density weights
+// grhom Omega_c / a, ..., w, c_a^2, cs2_DE, a itself, a m_nu) and, for
+// massive-nu rows, the bin velocity v_q = (1 + (a amnu / q)^2)^(-1/2), whose
+// partials are folded into those of a and amnu.  The derivative of
+// sum_rows sum_fields mul * coef is therefore sum_m Fbar_m dF_m, where Fbar_m
+// (plain doubles) are the partials of the row formulas below and dF_m the
+// features' tangents.
+enum : int {
+  FT_A = 0, FT_AH, FT_IAH, FT_OPAC, FT_CS2, FT_PB43, FT_RC, FT_RB, FT_RG, FT_RR, FT_RN, FT_RQ, FT_W, FT_CA2,
+  FT_CS2Q, FT_AMNU, FT_N
+};
+
+template <class T>
+__device__ __forceinline__ void features_of(const Ctx &cx, T a, const Bg<T> &b, T (&F)[FT_N]) {
+  using P = ParT<T>;
+  const P grhom = cv<T>(cx, F_GRHOM), grhog = cv<T>(cx, F_GRHOG), grhor = cv<T>(cx, F_GRHOR);
+  const P Omegam = cv<T>(cx, F_OMEGAM), Omegab = cv<T>(cx, F_OMEGAB), OmegaDE = cv<T>(cx, F_OMEGADE);
+  const P Neff = cv<T>(cx, F_NEFF), Nmnu = cv<T>(cx, F_NMNU);
+  const T a2 = a * a;
+  F[FT_A] = a;
+  F[FT_AH] = b.aH;
+  F[FT_IAH] = 1.0 / b.aH;
+  F[FT_OPAC] = b.opac;
+  F[FT_CS2] = b.cs2;
+  F[FT_PB43] = b.pb43;
+  F[FT_RC] = grhom * (Omegam - Omegab) / a;
+  F[FT_RB] = grhom * Omegab / a;
+  F[FT_RG] = grhog / a2;
+  F[FT_RR] = grhor * Neff / a2;
+  F[FT_RN] = grhor * Nmnu / a2;
+  F[FT_RQ] = grhom * OmegaDE * b.rhoQ * a2;
+  F[FT_W] = b.w;
+  F[FT_CA2] = b.ca2;
+  F[FT_CS2Q] = tconst<T>(0.0) + cv<T>(cx, F_CS2DE);
+  F[FT_AMNU] = tconst<T>(0.0) + cv<T>(cx, F_AMNU);
+}
+
+template <class T>
+__device__ __forceinline__ void features_t(Ctx &cx, T a, T (&F)[FT_N]) {
+  const Bg<T> b = background<T>(cx, a);
+  features_of<T>(cx, a, b, F);
+}
+
 // Values of the functionals of Y and of the "where they enter" sums of w that
 // the first-order contraction below needs, plus (optionally) the transposed
 // right-hand side A(a)^T w -- one plain-double pass over the rows.
@@ -280,9 +327,10 @@ struct RowSums {
 
 template <int NS, bool TAPPLY>
 DEB_OUTLINE RowSums coef_pass(Ctx &cx, const uint32_t (&rd)[NS], double a, double invtau,
-                                             const double (&w)[NS], const double (&Y)[NS], double (&out)[NS]) {
+                              const double (&w)[NS], const double (&Y)[NS], double (&out)[NS], double (&fv)[FT_N]) {
   const int lane = lane_id();
-  Bg<double> b = background<double>(cx, a);
+  const Bg<double> b = background<double>(cx, a);
+  features_of<double>(cx, a, b, fv);
   double part[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   double cc[4][NS];
 #pragma unroll
@@ -319,235 +367,279 @@ DEB_OUTLINE RowSums coef_pass(Ctx &cx, const uint32_t (&rd)[NS], double a, doubl
   return rs;
 }
 
-// ---------------------------------------------------------------------------
-// First-order contraction through background features.
-//
-// Every coefficient of row_visit depends on (a, theta) only through a few
-// wave-uniform features (aH, 1/aH, opac, cs2, pb43, the density weights
-// grhom Omega_c / a, ..., w, c_a^2, cs2_DE, a itself, a m_nu) and, for
-// massive-nu rows, the bin velocity v_q = (1 + (a amnu / q)^2)^(-1/2), whose
-// partials are folded into those of a and amnu.  The derivative of
-// sum_rows sum_fields mul * coef is therefore sum_m Fbar_m dF_m, where Fbar_m
-// (plain doubles) are the partials of the row formulas below and dF_m the
-// features' tangents.
-enum : int {
-  FT_A = 0, FT_AH, FT_IAH, FT_OPAC, FT_CS2, FT_PB43, FT_RC, FT_RB, FT_RG, FT_RR, FT_RN, FT_RQ, FT_W, FT_CA2,
-  FT_CS2Q, FT_AMNU, FT_N
-};
-
-template <class T>
-__device__ __forceinline__ void features_t(Ctx &cx, T a, T (&F)[FT_N]) {
-  using P = ParT<T>;
-  Bg<T> b = background<T>(cx, a);
-  const P grhom = cv<T>(cx, F_GRHOM), grhog = cv<T>(cx, F_GRHOG), grhor = cv<T>(cx, F_GRHOR);
-  const P Omegam = cv<T>(cx, F_OMEGAM), Omegab = cv<T>(cx, F_OMEGAB), OmegaDE = cv<T>(cx, F_OMEGADE);
-  const P Neff = cv<T>(cx, F_NEFF), Nmnu = cv<T>(cx, F_NMNU);
-  const T a2 = a * a;
-  F[FT_A] = a;
-  F[FT_AH] = b.aH;
-  F[FT_IAH] = 1.0 / b.aH;
-  F[FT_OPAC] = b.opac;
-  F[FT_CS2] = b.cs2;
-  F[FT_PB43] = b.pb43;
-  F[FT_RC] = grhom * (Omegam - Omegab) / a;
-  F[FT_RB] = grhom * Omegab / a;
-  F[FT_RG] = grhog / a2;
-  F[FT_RR] = grhor * Neff / a2;
-  F[FT_RN] = grhor * Nmnu / a2;
-  F[FT_RQ] = grhom * OmegaDE * b.rhoQ * a2;
-  F[FT_W] = b.w;
-  F[FT_CA2] = b.ca2;
-  F[FT_CS2Q] = tconst<T>(0.0) + cv<T>(cx, F_CS2DE);
-  F[FT_AMNU] = tconst<T>(0.0) + cv<T>(cx, F_AMNU);
-}
-
-// partials of one row's sum_fields mul[f] coef_f with respect to the features
-// (accumulated into fb) and to its bin velocity v (returned); fv: feature values
-__device__ __forceinline__ double row_feature_adj(uint32_t d, const Ctx &cx, const double (&fv)[FT_N], double vq,
-                                                  const double (&mul)[11], double (&fb)[FT_N]) {
+// partials of one row's sum_fields mul[q][f] coef_f with respect to the
+// features (accumulated into fb[q]) and to its bin velocity (vb[q]), for K
+// contractions at once (one pass of the divergent row switch); fv[q]: the
+// feature values of contraction q
+template <int K>
+__device__ __forceinline__ void row_feature_adj(uint32_t d, const Ctx &cx, const double (&fv)[K][FT_N],
+                                                const double (&vq)[K], const double (&mul)[K][11],
+                                                double (&fb)[K][FT_N], double (&vb)[K]) {
   const double k = cx.k, k2 = k * k;
   const uint32_t t = rd_type(d);
   const double l = (double)rd_l(d);
   const int qi = (int)rd_q(d);
-  const double a = fv[FT_A], aH = fv[FT_AH], iaH = fv[FT_IAH], opac = fv[FT_OPAC], pb43 = fv[FT_PB43];
-  const double m_sub = mul[C_SUB], m_diag = mul[C_DIAG], m_sup = mul[C_SUP], m_uh = mul[C_UH];
-  const double m_upi = mul[C_UPI], m_ch = mul[C_CH], m_ceta = mul[C_CETA], m_c1 = mul[C_C1];
-  double vb = 0.0;
   switch (t) {
     case R_ETA:
-      fb[FT_IAH] += m_ch * 2.0 * k2;
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        fb[q][FT_IAH] += mul[q][C_CH] * 2.0 * k2;
+      }
       break;
     case R_DC:
-      fb[FT_RC] += m_ch * iaH - m_c1 * a;
-      fb[FT_IAH] += m_ch * fv[FT_RC];
-      fb[FT_A] += -m_c1 * fv[FT_RC];
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        fb[q][FT_RC] += mul[q][C_CH] * fv[q][FT_IAH] - mul[q][C_C1] * fv[q][FT_A];
+        fb[q][FT_IAH] += mul[q][C_CH] * fv[q][FT_RC];
+        fb[q][FT_A] += -mul[q][C_C1] * fv[q][FT_RC];
+      }
       break;
     case R_DB:
-      fb[FT_RB] += m_ch * iaH - m_c1 * a;
-      fb[FT_IAH] += m_ch * fv[FT_RB];
-      fb[FT_A] += -m_c1 * fv[FT_RB];
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        fb[q][FT_RB] += mul[q][C_CH] * fv[q][FT_IAH] - mul[q][C_C1] * fv[q][FT_A];
+        fb[q][FT_IAH] += mul[q][C_CH] * fv[q][FT_RB];
+        fb[q][FT_A] += -mul[q][C_C1] * fv[q][FT_RB];
+      }
       break;
     case R_TC:
-      fb[FT_AH] += -m_diag;
-      fb[FT_RC] += m_ceta * (0.5 / k2);
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        fb[q][FT_AH] += -mul[q][C_DIAG];
+        fb[q][FT_RC] += mul[q][C_CETA] * (0.5 / k2);
+      }
       break;
     case R_TB:
-      fb[FT_CS2] += m_sub * k2;
-      fb[FT_AH] += -m_diag;
-      fb[FT_PB43] += (m_sup - m_diag) * opac;
-      fb[FT_OPAC] += (m_sup - m_diag) * pb43;
-      fb[FT_RB] += m_ceta * (0.5 / k2);
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        fb[q][FT_CS2] += mul[q][C_SUB] * k2;
+        fb[q][FT_AH] += -mul[q][C_DIAG];
+        fb[q][FT_PB43] += (mul[q][C_SUP] - mul[q][C_DIAG]) * fv[q][FT_OPAC];
+        fb[q][FT_OPAC] += (mul[q][C_SUP] - mul[q][C_DIAG]) * fv[q][FT_PB43];
+        fb[q][FT_RB] += mul[q][C_CETA] * (0.5 / k2);
+      }
       break;
     case R_DG:
-      fb[FT_RG] += m_ch * iaH - 2.0 * m_c1 * a;
-      fb[FT_IAH] += m_ch * fv[FT_RG];
-      fb[FT_A] += -2.0 * m_c1 * fv[FT_RG];
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        fb[q][FT_RG] += mul[q][C_CH] * fv[q][FT_IAH] - 2.0 * mul[q][C_C1] * fv[q][FT_A];
+        fb[q][FT_IAH] += mul[q][C_CH] * fv[q][FT_RG];
+        fb[q][FT_A] += -2.0 * mul[q][C_C1] * fv[q][FT_RG];
+      }
       break;
     case R_TG:
-      fb[FT_OPAC] += m_sub - m_diag;
-      fb[FT_RG] += m_ceta * (4.0 / 3.0) * (0.5 / k2);
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        fb[q][FT_OPAC] += mul[q][C_SUB] - mul[q][C_DIAG];
+        fb[q][FT_RG] += mul[q][C_CETA] * (4.0 / 3.0) * (0.5 / k2);
+      }
       break;
     case R_F2:
-      fb[FT_OPAC] += -m_diag + 0.1 * m_upi;
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        fb[q][FT_OPAC] += -mul[q][C_DIAG] + 0.1 * mul[q][C_UPI];
+      }
       break;
     case R_G0:
-      fb[FT_OPAC] += -m_diag + 0.5 * m_upi;
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        fb[q][FT_OPAC] += -mul[q][C_DIAG] + 0.5 * mul[q][C_UPI];
+      }
       break;
     case R_G2:
-      fb[FT_OPAC] += -m_diag + 0.1 * m_upi;
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        fb[q][FT_OPAC] += -mul[q][C_DIAG] + 0.1 * mul[q][C_UPI];
+      }
       break;
     case R_FL:
     case R_FLMAX:
     case R_GL:
     case R_GLMAX:
-      fb[FT_OPAC] += -m_diag;
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        fb[q][FT_OPAC] += -mul[q][C_DIAG];
+      }
       break;
     case R_DR:
-      fb[FT_RR] += m_ch * iaH - 2.0 * m_c1 * a;
-      fb[FT_IAH] += m_ch * fv[FT_RR];
-      fb[FT_A] += -2.0 * m_c1 * fv[FT_RR];
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        fb[q][FT_RR] += mul[q][C_CH] * fv[q][FT_IAH] - 2.0 * mul[q][C_C1] * fv[q][FT_A];
+        fb[q][FT_IAH] += mul[q][C_CH] * fv[q][FT_RR];
+        fb[q][FT_A] += -2.0 * mul[q][C_C1] * fv[q][FT_RR];
+      }
       break;
     case R_TR:
-      fb[FT_RR] += m_ceta * (4.0 / 3.0) * (0.5 / k2);
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        fb[q][FT_RR] += mul[q][C_CETA] * (4.0 / 3.0) * (0.5 / k2);
+      }
       break;
     case R_Q0: {
-      const double wq = cx.w[qi], RN = fv[FT_RN];
-      const double iv = 1.0 / vq;
-      vb += -k * m_sup;
-      // ch = RN wq / v * iaH
-      fb[FT_RN] += m_ch * wq * iv * iaH;
-      fb[FT_IAH] += m_ch * RN * wq * iv;
-      vb += -m_ch * RN * wq * iaH * iv * iv;
-      // c1 = -a (RN wq / v + RN wq v)
-      fb[FT_A] += -m_c1 * RN * wq * (iv + vq);
-      fb[FT_RN] += -m_c1 * a * wq * (iv + vq);
-      vb += -m_c1 * a * RN * wq * (1.0 - iv * iv);
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        const double wq = cx.w[qi], RN = fv[q][FT_RN];
+        const double iv = 1.0 / vq[q];
+        vb[q] += -k * mul[q][C_SUP];
+        // ch = RN wq / v * iaH
+        fb[q][FT_RN] += mul[q][C_CH] * wq * iv * fv[q][FT_IAH];
+        fb[q][FT_IAH] += mul[q][C_CH] * RN * wq * iv;
+        vb[q] += -mul[q][C_CH] * RN * wq * fv[q][FT_IAH] * iv * iv;
+        // c1 = -a (RN wq / v + RN wq v)
+        fb[q][FT_A] += -mul[q][C_C1] * RN * wq * (iv + vq[q]);
+        fb[q][FT_RN] += -mul[q][C_C1] * fv[q][FT_A] * wq * (iv + vq[q]);
+        vb[q] += -mul[q][C_C1] * fv[q][FT_A] * RN * wq * (1.0 - iv * iv);
+      }
       break;
     }
     case R_Q1:
-      vb += k * (m_sub / 3.0 - 2.0 * m_sup / 3.0);
-      fb[FT_RN] += m_ceta * k * cx.w[qi] * (0.5 / k2);
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        vb[q] += k * (mul[q][C_SUB] / 3.0 - 2.0 * mul[q][C_SUP] / 3.0);
+        fb[q][FT_RN] += mul[q][C_CETA] * k * cx.w[qi] * (0.5 / k2);
+      }
       break;
     case R_Q2:
-      vb += k * (0.4 * m_sub - 0.6 * m_sup);
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        vb[q] += k * (0.4 * mul[q][C_SUB] - 0.6 * mul[q][C_SUP]);
+      }
       break;
     case R_QL:
-      vb += k * (m_sub * (l / (2.0 * l + 1.0)) - m_sup * ((l + 1.0) / (2.0 * l + 1.0)));
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        vb[q] += k * (mul[q][C_SUB] * (l / (2.0 * l + 1.0)) - mul[q][C_SUP] * ((l + 1.0) / (2.0 * l + 1.0)));
+      }
       break;
     case R_QLMAX:
-      vb += k * m_sub;
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        vb[q] += k * mul[q][C_SUB];
+      }
       break;
     case R_DQ:
-      if (!cx.lambda) {
-        const double w = fv[FT_W], ca2 = fv[FT_CA2], cs2Q = fv[FT_CS2Q], RQ = fv[FT_RQ];
-        const double opw = 1.0 + w, g = aH * aH / k2;
-        // diag = -3 (cs2Q - w) aH
-        fb[FT_CS2Q] += -3.0 * aH * m_diag;
-        fb[FT_W] += 3.0 * aH * m_diag;
-        fb[FT_AH] += -3.0 * (cs2Q - w) * m_diag;
-        // sup = -opw - 9 opw (cs2Q - ca2) aH^2 / k2
-        fb[FT_W] += m_sup * (-1.0 - 9.0 * (cs2Q - ca2) * g);
-        fb[FT_CS2Q] += -9.0 * opw * g * m_sup;
-        fb[FT_CA2] += 9.0 * opw * g * m_sup;
-        fb[FT_AH] += -18.0 * opw * (cs2Q - ca2) * aH / k2 * m_sup;
-        // uh = -0.5 opw
-        fb[FT_W] += -0.5 * m_uh;
-        // ch = RQ iaH
-        fb[FT_RQ] += m_ch * iaH;
-        fb[FT_IAH] += m_ch * RQ;
-        // c1 = -a (RQ + 3 cs2Q RQ)
-        fb[FT_A] += -m_c1 * RQ * (1.0 + 3.0 * cs2Q);
-        fb[FT_RQ] += -m_c1 * a * (1.0 + 3.0 * cs2Q);
-        fb[FT_CS2Q] += -m_c1 * 3.0 * a * RQ;
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        if (!cx.lambda) {
+          const double w = fv[q][FT_W], ca2 = fv[q][FT_CA2], cs2Q = fv[q][FT_CS2Q], RQ = fv[q][FT_RQ];
+          const double opw = 1.0 + w, g = fv[q][FT_AH] * fv[q][FT_AH] / k2;
+          // diag = -3 (cs2Q - w) aH
+          fb[q][FT_CS2Q] += -3.0 * fv[q][FT_AH] * mul[q][C_DIAG];
+          fb[q][FT_W] += 3.0 * fv[q][FT_AH] * mul[q][C_DIAG];
+          fb[q][FT_AH] += -3.0 * (cs2Q - w) * mul[q][C_DIAG];
+          // sup = -opw - 9 opw (cs2Q - ca2) aH^2 / k2
+          fb[q][FT_W] += mul[q][C_SUP] * (-1.0 - 9.0 * (cs2Q - ca2) * g);
+          fb[q][FT_CS2Q] += -9.0 * opw * g * mul[q][C_SUP];
+          fb[q][FT_CA2] += 9.0 * opw * g * mul[q][C_SUP];
+          fb[q][FT_AH] += -18.0 * opw * (cs2Q - ca2) * fv[q][FT_AH] / k2 * mul[q][C_SUP];
+          // uh = -0.5 opw
+          fb[q][FT_W] += -0.5 * mul[q][C_UH];
+          // ch = RQ iaH
+          fb[q][FT_RQ] += mul[q][C_CH] * fv[q][FT_IAH];
+          fb[q][FT_IAH] += mul[q][C_CH] * RQ;
+          // c1 = -a (RQ + 3 cs2Q RQ)
+          fb[q][FT_A] += -mul[q][C_C1] * RQ * (1.0 + 3.0 * cs2Q);
+          fb[q][FT_RQ] += -mul[q][C_C1] * fv[q][FT_A] * (1.0 + 3.0 * cs2Q);
+          fb[q][FT_CS2Q] += -mul[q][C_C1] * 3.0 * fv[q][FT_A] * RQ;
+        }
       }
       break;
     case R_TQ:
-      if (!cx.lambda) {
-        const double w = fv[FT_W], ca2 = fv[FT_CA2], cs2Q = fv[FT_CS2Q], RQ = fv[FT_RQ];
-        const double opw = 1.0 + w;
-        // sub = cs2Q / opw k2
-        fb[FT_CS2Q] += m_sub * k2 / opw;
-        fb[FT_W] += -m_sub * cs2Q * k2 / (opw * opw);
-        // diag = -(1 - 3 cs2Q) aH
-        fb[FT_CS2Q] += 3.0 * aH * m_diag;
-        fb[FT_AH] += -(1.0 - 3.0 * cs2Q) * m_diag;
-        // ceta = opw RQ 0.5 / k2
-        fb[FT_W] += m_ceta * RQ * (0.5 / k2);
-        fb[FT_RQ] += m_ceta * opw * (0.5 / k2);
-        // c1 = -a 9 (cs2Q - ca2) aH opw RQ / k2
-        const double c9 = 9.0 / k2;
-        fb[FT_A] += -m_c1 * c9 * (cs2Q - ca2) * aH * opw * RQ;
-        fb[FT_CS2Q] += -m_c1 * a * c9 * aH * opw * RQ;
-        fb[FT_CA2] += m_c1 * a * c9 * aH * opw * RQ;
-        fb[FT_AH] += -m_c1 * a * c9 * (cs2Q - ca2) * opw * RQ;
-        fb[FT_W] += -m_c1 * a * c9 * (cs2Q - ca2) * aH * RQ;
-        fb[FT_RQ] += -m_c1 * a * c9 * (cs2Q - ca2) * aH * opw;
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        if (!cx.lambda) {
+          const double w = fv[q][FT_W], ca2 = fv[q][FT_CA2], cs2Q = fv[q][FT_CS2Q], RQ = fv[q][FT_RQ];
+          const double opw = 1.0 + w;
+          // sub = cs2Q / opw k2
+          fb[q][FT_CS2Q] += mul[q][C_SUB] * k2 / opw;
+          fb[q][FT_W] += -mul[q][C_SUB] * cs2Q * k2 / (opw * opw);
+          // diag = -(1 - 3 cs2Q) aH
+          fb[q][FT_CS2Q] += 3.0 * fv[q][FT_AH] * mul[q][C_DIAG];
+          fb[q][FT_AH] += -(1.0 - 3.0 * cs2Q) * mul[q][C_DIAG];
+          // ceta = opw RQ 0.5 / k2
+          fb[q][FT_W] += mul[q][C_CETA] * RQ * (0.5 / k2);
+          fb[q][FT_RQ] += mul[q][C_CETA] * opw * (0.5 / k2);
+          // c1 = -a 9 (cs2Q - ca2) aH opw RQ / k2
+          const double c9 = 9.0 / k2;
+          fb[q][FT_A] += -mul[q][C_C1] * c9 * (cs2Q - ca2) * fv[q][FT_AH] * opw * RQ;
+          fb[q][FT_CS2Q] += -mul[q][C_C1] * fv[q][FT_A] * c9 * fv[q][FT_AH] * opw * RQ;
+          fb[q][FT_CA2] += mul[q][C_C1] * fv[q][FT_A] * c9 * fv[q][FT_AH] * opw * RQ;
+          fb[q][FT_AH] += -mul[q][C_C1] * fv[q][FT_A] * c9 * (cs2Q - ca2) * opw * RQ;
+          fb[q][FT_W] += -mul[q][C_C1] * fv[q][FT_A] * c9 * (cs2Q - ca2) * fv[q][FT_AH] * RQ;
+          fb[q][FT_RQ] += -mul[q][C_C1] * fv[q][FT_A] * c9 * (cs2Q - ca2) * fv[q][FT_AH] * opw;
+        }
       }
       break;
     default:
       break;
   }
-  return vb;
 }
 
 // First-order contraction of sum_rows w_row f_row(a, Y) along (a, theta):
 //   feature_partials per lane, the partials of its rows' sum_fields mul * coef
 //                    with respect to the features, accumulated into fb (plain
-//                    doubles, no cross-lane sums);
+//                    doubles, no cross-lane sums), for K contractions in one
+//                    pass over the rows;
 //   lane_dot         after fb is summed over the wave, sum_m dF_m fb_m for the
 //                    lane's own tangent direction (lane_ctx).
 // The functionals (h', eta', Pi, f_1) enter through their double values in rs,
 // which is exact at first order: d(u H) = du H + u dH with dH = sum_rows dc_h Y,
 // so sum_rows w u dH = s_h sum_rows dc_h Y (multiplier Y s_h of the c_h field).
-template <int NS>
-DEB_OUTLINE void feature_partials(const Ctx &cx, const uint32_t (&rd)[NS], const double (&fv)[FT_N],
-                                  const double (&Y)[NS], const double (&w)[NS], const RowSums &rs,
-                                  double (&fb)[FT_N]) {
+template <int NS, int K>
+DEB_OUTLINE void feature_partials(const Ctx &cx, const uint32_t (&rd)[NS], const double (&fv)[K][FT_N],
+                                  const double (&Y)[K][NS], const double (&w)[K][NS], const RowSums (&rs)[K],
+                                  double (&fb)[K][FT_N]) {
   const int lane = lane_id();
-  const double ydg = readlane(Y[0], L_DG), ytg = readlane(Y[0], L_TG);
-  const double wahp = readlane(w[0], L_AHP);
+  double ydg[K], ytg[K], wahp[K];
+#pragma unroll
+  for (int q = 0; q < K; ++q) {
+    ydg[q] = readlane(Y[q][0], L_DG);
+    ytg[q] = readlane(Y[q][0], L_TG);
+    wahp[q] = readlane(w[q][0], L_AHP);
+  }
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
-    const double ym = from_prev(Y[s]), yp = from_next(Y[s]);
-    const double lp = (s == 0) ? (lane == L_DG ? ytg : (lane == L_TG ? ydg : 0.0)) : 0.0;
-    const double mul[11] = {w[s] * ym,    w[s] * Y[s],  w[s] * yp,    w[s] * rs.H,  w[s] * rs.E, w[s] * rs.Pi,
-                            Y[s] * rs.sh, Y[s] * rs.se, Y[s] * rs.sp, Y[s] * wahp, w[s] * lp};
     const uint32_t ty = rd_type(rd[s]);
     const bool qrow = ty >= R_Q0 && ty <= R_QLMAX;
-    // v_q(x), x = a amnu / q: dv/dx = -x v^3
     const double iq = qrow ? 1.0 / cx.q[rd_q(rd[s])] : 0.0;
-    const double x = fv[FT_A] * fv[FT_AMNU] * iq;
-    const double v = 1.0 / sqrt(1.0 + x * x);
-    const double vb = row_feature_adj(rd[s], cx, fv, v, mul, fb);
+    double mul[K][11], vq[K], xq[K], vb[K];
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+      const double ys = Y[q][s], ws = w[q][s];
+      const double ym = from_prev(ys), yp = from_next(ys);
+      const double lp = (s == 0) ? (lane == L_DG ? ytg[q] : (lane == L_TG ? ydg[q] : 0.0)) : 0.0;
+      mul[q][C_SUB] = ws * ym;
+      mul[q][C_DIAG] = ws * ys;
+      mul[q][C_SUP] = ws * yp;
+      mul[q][C_UH] = ws * rs[q].H;
+      mul[q][C_UETA] = ws * rs[q].E;
+      mul[q][C_UPI] = ws * rs[q].Pi;
+      mul[q][C_CH] = ys * rs[q].sh;
+      mul[q][C_CETA] = ys * rs[q].se;
+      mul[q][C_CPI] = ys * rs[q].sp;
+      mul[q][C_C1] = ys * wahp[q];
+      mul[q][C_LEAF] = ws * lp;
+      // v_q(x), x = a amnu / q
+      xq[q] = fv[q][FT_A] * fv[q][FT_AMNU] * iq;
+      vq[q] = 1.0 / sqrt(1.0 + xq[q] * xq[q]);
+      vb[q] = 0.0;
+    }
+    row_feature_adj<K>(rd[s], cx, fv, vq, mul, fb, vb);
     if (qrow) {
-      const double dvdx = -x * v * v * v;
-      fb[FT_A] += vb * dvdx * fv[FT_AMNU] * iq;
-      fb[FT_AMNU] += vb * dvdx * fv[FT_A] * iq;
+#pragma unroll
+      for (int q = 0; q < K; ++q) {
+        const double dvdx = -xq[q] * vq[q] * vq[q] * vq[q];
+        fb[q][FT_A] += vb[q] * dvdx * fv[q][FT_AMNU] * iq;
+        fb[q][FT_AMNU] += vb[q] * dvdx * fv[q][FT_A] * iq;
+      }
     }
   }
   if (lane == L_A) {  // f_a = aH a
-    fb[FT_AH] += w[0] * fv[FT_A];
-    fb[FT_A] += w[0] * fv[FT_AH];
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+      fb[q][FT_AH] += w[q][0] * fv[q][FT_A];
+      fb[q][FT_A] += w[q][0] * fv[q][FT_AH];
+    }
   }
 }
 
@@ -689,12 +781,14 @@ __device__ __forceinline__ void rodas5_step_vjp(Mode<NS> &m, Ctx &cx, double t0,
 #pragma unroll
   for (int j = 0; j < MAXDIR; ++j) th_acc[j] = 0.0;
   double a0b = 0.0, t0b = 0.0, hb = 0.0;
-  // J(t0, y0) k_i terms of all stages share the features at a0: their
-  // partials are summed over the stages and contracted once after the loop
-  double fv0[FT_N], fbJ[FT_N];
-  feature_values(cx, a0, fv0);
+  // Feature partials of each stage, in one pass over the rows per stage:
+  // set 0 for f(t_i, u_i) (features at a_i), set 1 for the J(t0, y0) k_i term,
+  // whose features at a0 are shared by all stages: its partials are summed
+  // over the stages and contracted once after the loop.
+  double fvk[2][FT_N], fbk[2][FT_N];
+  feature_values(cx, a0, fvk[1]);
 #pragma unroll
-  for (int q = 0; q < FT_N; ++q) fbJ[q] = 0.0;
+  for (int q = 0; q < FT_N; ++q) fbk[1][q] = 0.0;
 #pragma nounroll
   for (int i = 7; i >= 0; --i) {
     double rb[NS], ki[NS];
@@ -739,24 +833,36 @@ __device__ __forceinline__ void rodas5_step_vjp(Mode<NS> &m, Ctx &cx, double t0,
 #pragma unroll
       for (int s = 0; s < NS; ++s) dTb[s] += kd[i] * h * rb[s];
     }
-    // f(t_i, u_i): ubar = J(t_i, u_i)^T rbar
+    // f(t_i, u_i): ubar = J(t_i, u_i)^T rbar, and the J(t0, y0) k_i term
+    // (its y-linear part at a0; the a-column goes through rho)
     double ub[NS];
     {
       const double ai = readlane(u[0], L_A);
-      const RowSums rs = coef_pass<NS, true>(cx, m.rd, ai, iti, rb, u, ub);
-      double fv[FT_N], fb[FT_N];
-      feature_values(cx, ai, fv);
+      double Yk[2][NS], wk[2][NS];
+      RowSums rsk[2];
+      rsk[0] = coef_pass<NS, true>(cx, m.rd, ai, iti, rb, u, ub, fvk[0]);
 #pragma unroll
-      for (int q = 0; q < FT_N; ++q) fb[q] = 0.0;
-      feature_partials<NS>(cx, m.rd, fv, u, rb, rs, fb);
-      wave_sum<FT_N>(fb);
-      const double g = lane_contract(cx, ai, fb);
+      for (int s = 0; s < NS; ++s) {
+        Yk[0][s] = u[s];
+        wk[0][s] = rb[s];
+        Yk[1][s] = ki[s];
+        wk[1][s] = (s == 0 && lane == L_A) ? 0.0 : rb[s];
+      }
+      rsk[1] = jac_sums<NS>(m, Yk[1], wk[1]);
+#pragma unroll
+      for (int q = 0; q < FT_N; ++q) fbk[0][q] = 0.0;
+      feature_partials<NS, 2>(cx, m.rd, fvk, Yk, wk, rsk, fbk);
+      wave_sum<FT_N>(fbk[0]);
+      const double g = lane_contract(cx, ai, fbk[0]);
       const double ga = readlane(g, 0);
       if (lane == L_A) ub[0] += ga;
       double tl[MAXDIR];
       lane_gather(g, cx.ndir, tl);
 #pragma unroll
       for (int j = 0; j < MAXDIR; ++j) th_acc[j] += tl[j];
+      const double kia = readlane(ki[0], L_A);
+#pragma unroll
+      for (int s = 0; s < NS; ++s) rho[s] += kia * rb[s];
     }
 #pragma unroll
     for (int s = 0; s < NS; ++s) y0b[s] += ub[s];
@@ -765,23 +871,11 @@ __device__ __forceinline__ void rodas5_step_vjp(Mode<NS> &m, Ctx &cx, double t0,
 #pragma unroll
       for (int s = 0; s < NS; ++s) KBp(j, s) += aij * ub[s];
     }
-    // J(t0, y0) k_i: the y-linear part at a0 (the a-column goes through rho)
-    {
-      double wr[NS];
-#pragma unroll
-      for (int s = 0; s < NS; ++s) wr[s] = rb[s];
-      if (lane == L_A) wr[0] = 0.0;
-      const RowSums rs = jac_sums<NS>(m, ki, wr);
-      feature_partials<NS>(cx, m.rd, fv0, ki, wr, rs, fbJ);
-      const double kia = readlane(ki[0], L_A);
-#pragma unroll
-      for (int s = 0; s < NS; ++s) rho[s] += kia * rb[s];
-    }
   }
   // the J(t0, y0) k_i terms, contracted with the feature tangents at a0
   {
-    wave_sum<FT_N>(fbJ);
-    const double g = lane_contract(cx, a0, fbJ);
+    wave_sum<FT_N>(fbk[1]);
+    const double g = lane_contract(cx, a0, fbk[1]);
     a0b += readlane(g, 0);
     double tl[MAXDIR];
     lane_gather(g, cx.ndir, tl);

@@ -68,14 +68,19 @@ __global__ void __launch_bounds__(WAVE) deb_rhs_vjp_kernel(const Params<NS> P, c
   }
   const double tau = H.t0[item], it = 1.0 / tau;
   const double a = readlane(y[0], L_A);
-  const RowSums rs = coef_pass<NS, true>(cx, rd, a, it, w, y, yb);
-  double fv[FT_N], fb[FT_N];
-  feature_values(cx, a, fv);
+  double fv[1][FT_N], fb[1][FT_N], Yk[1][NS], wk[1][NS];
+  RowSums rs[1];
+  rs[0] = coef_pass<NS, true>(cx, rd, a, it, w, y, yb, fv[0]);
 #pragma unroll
-  for (int q = 0; q < FT_N; ++q) fb[q] = 0.0;
-  feature_partials<NS>(cx, rd, fv, y, w, rs, fb);
-  wave_sum<FT_N>(fb);
-  const double gl = lane_contract(cx, a, fb);
+  for (int s = 0; s < NS; ++s) {
+    Yk[0][s] = y[s];
+    wk[0][s] = w[s];
+  }
+#pragma unroll
+  for (int q = 0; q < FT_N; ++q) fb[0][q] = 0.0;
+  feature_partials<NS, 1>(cx, rd, fv, Yk, wk, rs, fb);
+  wave_sum<FT_N>(fb[0]);
+  const double gl = lane_contract(cx, a, fb[0]);
   double gth[MAXDIR];
   lane_gather(gl, H.ndir, gth);
   const double ga = readlane(gl, 0);
