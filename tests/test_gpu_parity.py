@@ -136,18 +136,26 @@ def _rel(a, b, axis_scale):
     return np.abs(a - b) / sc
 
 
+# (final, intermediate) output tolerances at rtol 1e-10: about twice the
+# oracle's own change between rtol 1e-10 and 1e-12 over these k (final /
+# intermediate: matter and metric fields 4e-11 / 1.1e-5, delta_gamma 1.7e-8 /
+# 1.9e-4, delta_r 4.3e-6 / 7.7e-4; intermediate outputs are linearly
+# interpolated between accepted steps, as the reference's saved points are)
+TIGHT_TOL = {12: (5e-8, 4e-4), 14: (1e-5, 1.5e-3)}
+
+
 def test_evolve_parity_tight_tolerance(gpu, test_param):
-    """At rtol 1e-10 step-sequence differences no longer matter: the final
-    output agrees to 1e-9; interpolated outputs of the smallest k (a few huge
-    steps) keep a linear-interpolation sensitivity (measured <= 2.4e-6)."""
+    """At rtol 1e-10 the GPU solve and the oracle agree to within the
+    solution's own tolerance sensitivity, field by field (TIGHT_TOL)."""
     k = np.geomspace(1e-4, 0.5, 6)
     ref, y, st, ns = _evolve_both(test_param, CONFIGS["ref-default"], k, [0.01, 0.1, 1.0], 1e-10, 1e-14,
                                   max_steps=400000)
     assert np.all(st == 0) and np.all(ref["status"] == 0)
     for f in DELTA_FIELDS + (0, 2, 5, 7):
+        tf, ti = TIGHT_TOL.get(f, (1e-9, 2e-5))
         rel = _rel(y[:, :, f], ref["y"][:, :, f], 1)
-        assert rel[:, -1].max() < 1e-9, (f, rel[:, -1].max())
-        assert rel.max() < 1e-5, (f, rel.max())
+        assert rel[:, -1].max() < tf, (f, rel[:, -1].max())
+        assert rel.max() < ti, (f, rel.max())
 
 
 def _check_final(ref, y, ns, rtol):
