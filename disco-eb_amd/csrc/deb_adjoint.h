@@ -157,9 +157,10 @@ __device__ __forceinline__ void bsolveT(const Mode<NS> &m, double (&x)[NS]) {
   if (lane == L_DG) x[0] += rdgb;
 }
 
-// x <- W^{-T} x (transpose of wsolve)
+// x <- W^{-T} x (transpose of wsolve); a call, so that the step factors stay
+// in memory rather than in registers across the reverse stage loop
 template <int NS>
-__device__ __forceinline__ void wsolveT(const Mode<NS> &m, double (&x)[NS]) {
+DEB_OUTLINE void wsolveT(const Mode<NS> &m, double (&x)[NS]) {
   const int lane = lane_id();
   const double kba = readlane(x[0], L_A), kb1 = readlane(x[0], L_AHP);
   if (lane == L_A || lane == L_AHP) x[0] = 0.0;
