@@ -732,17 +732,19 @@ __device__ __forceinline__ TD seed_td(double v, double d0) {
 }
 
 // the forward step from (t0, y0) again: factors into m, stage vectors into Kl
-// (inlined: as a call it is large enough to need long branches, which must
+// (inlined, with its factorisation, right-hand sides and solves as calls: the
+// whole step as one call is large enough to need long branches, which must
 // stay out of non-kernel functions)
 template <int NS>
 __device__ __forceinline__ void recompute_step(Mode<NS> &m, Ctx &cx, double t0, double h, const double (&y0)[NS],
                                                double *Kl) {
   const int lane = lane_id();
   double y1[NS], err[NS];
-  rodas5_step_s<NS>(m, cx, t0, h, y0, y1, err, [&](int i, const double (&k)[NS]) {
+  auto store = [&](int i, const double (&k)[NS]) {
 #pragma unroll
     for (int s = 0; s < NS; ++s) Kl[(i * NS + s) * WAVE + lane] = k[s];
-  });
+  };
+  rodas5_step_s<NS, decltype(store), true>(m, cx, t0, h, y0, y1, err, store);
 }
 
 // ---------------------------------------------------------------------------

@@ -328,91 +328,118 @@ struct R5 {
                           d4 = 0.3449828624725343, d5 = -0.377417564392089818;
 };
 
+// Out-of-line versions of the step's three parts, for kernels that run the
+// step next to much other live state (the reverse kernel): as calls, the step
+// factors stay in memory instead of registers.
+template <int NS>
+__device__ __noinline__ void factor_call(Mode<NS> &m, Ctx &cx, double tau0, const double (&y0)[NS], double h) {
+  factor<NS>(m, cx, tau0, y0, h);
+}
+template <int NS>
+__device__ __noinline__ void rhs_call(Ctx &cx, const uint32_t (&rd)[NS], double tau, const double (&y)[NS],
+                                      double (&f)[NS]) {
+  eval_rhs<NS, double>(cx, rd, tau, y, f);
+}
+template <int NS>
+__device__ __noinline__ void wsolve_call(const Mode<NS> &m, double (&r)[NS]) {
+  wsolve<NS>(m, r);
+}
+
 // one Rodas5Transformed step (ode_integrators_stiff.py:718-834): y1 and err = k8;
-// st(i, k) sees every stage vector k_{i+1} once it is solved
-template <int NS, class Store>
+// st(i, k) sees every stage vector k_{i+1} once it is solved.  CALLS selects
+// the out-of-line factorisation, right-hand side and solve.
+template <int NS, class Store, bool CALLS = false>
 __device__ __forceinline__ void rodas5_step_s(Mode<NS> &m, Ctx &cx, double t0, double dt, const double (&y0)[NS],
                                               double (&y1)[NS], double (&err)[NS], Store st) {
   using R = R5;
-  factor<NS>(m, cx, t0, y0, dt);
+  auto eval_rhs_ = [&](double tau, const double (&y)[NS], double (&f)[NS]) {
+    if constexpr (CALLS) rhs_call<NS>(cx, m.rd, tau, y, f);
+    else eval_rhs<NS, double>(cx, m.rd, tau, y, f);
+  };
+  auto wsolve_ = [&](double (&r)[NS]) {
+    if constexpr (CALLS) wsolve_call<NS>(m, r);
+    else wsolve<NS>(m, r);
+  };
+  if constexpr (CALLS) factor_call<NS>(m, cx, t0, y0, dt);
+  else factor<NS>(m, cx, t0, y0, dt);
   double dT[NS];
   dfdtau<NS>(m.rd, t0, y0, dT);
   double k1[NS], k2[NS], k3[NS], k4[NS], k5[NS], k6[NS], k7[NS];
   double u[NS], f[NS];
   const double idt = 1.0 / dt;
   // stage 1
-  eval_rhs<NS, double>(cx, m.rd, t0, y0, f);
+  eval_rhs_(t0, y0, f);
 #pragma unroll
   for (int s = 0; s < NS; ++s) k1[s] = f[s] + dt * R::d1 * dT[s];
-  wsolve<NS>(m, k1);
+  wsolve_(k1);
   st(0, k1);
   // stage 2
 #pragma unroll
   for (int s = 0; s < NS; ++s) u[s] = y0[s] + R::a21 * k1[s];
-  eval_rhs<NS, double>(cx, m.rd, t0 + R::c2 * dt, u, f);
+  eval_rhs_(t0 + R::c2 * dt, u, f);
 #pragma unroll
   for (int s = 0; s < NS; ++s) k2[s] = f[s] + dt * R::d2 * dT[s] + (R::C21 * idt) * k1[s];
-  wsolve<NS>(m, k2);
+  wsolve_(k2);
   st(1, k2);
   // stage 3
 #pragma unroll
   for (int s = 0; s < NS; ++s) u[s] = y0[s] + R::a31 * k1[s] + R::a32 * k2[s];
-  eval_rhs<NS, double>(cx, m.rd, t0 + R::c3 * dt, u, f);
+  eval_rhs_(t0 + R::c3 * dt, u, f);
 #pragma unroll
   for (int s = 0; s < NS; ++s)
     k3[s] = f[s] + dt * R::d3 * dT[s] + ((R::C31 * idt) * k1[s] + (R::C32 * idt) * k2[s]);
-  wsolve<NS>(m, k3);
+  wsolve_(k3);
   st(2, k3);
   // stage 4
 #pragma unroll
   for (int s = 0; s < NS; ++s) u[s] = y0[s] + R::a41 * k1[s] + R::a42 * k2[s] + R::a43 * k3[s];
-  eval_rhs<NS, double>(cx, m.rd, t0 + R::c4 * dt, u, f);
+  eval_rhs_(t0 + R::c4 * dt, u, f);
 #pragma unroll
   for (int s = 0; s < NS; ++s)
     k4[s] = f[s] + dt * R::d4 * dT[s] + ((R::C41 * idt) * k1[s] + (R::C42 * idt) * k2[s] + (R::C43 * idt) * k3[s]);
-  wsolve<NS>(m, k4);
+  wsolve_(k4);
   st(3, k4);
   // stage 5
 #pragma unroll
   for (int s = 0; s < NS; ++s) u[s] = y0[s] + R::a51 * k1[s] + R::a52 * k2[s] + R::a53 * k3[s] + R::a54 * k4[s];
-  eval_rhs<NS, double>(cx, m.rd, t0 + R::c5 * dt, u, f);
+  eval_rhs_(t0 + R::c5 * dt, u, f);
 #pragma unroll
   for (int s = 0; s < NS; ++s)
     k5[s] = f[s] + dt * R::d5 * dT[s] +
             ((R::C51 * idt) * k1[s] + (R::C52 * idt) * k2[s] + (R::C53 * idt) * k3[s] + (R::C54 * idt) * k4[s]);
-  wsolve<NS>(m, k5);
+  wsolve_(k5);
   st(4, k5);
   // stage 6
 #pragma unroll
   for (int s = 0; s < NS; ++s)
     u[s] = y0[s] + R::a61 * k1[s] + R::a62 * k2[s] + R::a63 * k3[s] + R::a64 * k4[s] + R::a65 * k5[s];
-  eval_rhs<NS, double>(cx, m.rd, t0 + dt, u, f);
+  eval_rhs_(t0 + dt, u, f);
 #pragma unroll
   for (int s = 0; s < NS; ++s)
     k6[s] = f[s] + ((R::C61 * idt) * k1[s] + (R::C62 * idt) * k2[s] + (R::C63 * idt) * k3[s] +
                     (R::C64 * idt) * k4[s] + (R::C65 * idt) * k5[s]);
-  wsolve<NS>(m, k6);
+  wsolve_(k6);
   st(5, k6);
   // stage 7
 #pragma unroll
   for (int s = 0; s < NS; ++s) u[s] = u[s] + k6[s];
-  eval_rhs<NS, double>(cx, m.rd, t0 + dt, u, f);
+  eval_rhs_(t0 + dt, u, f);
 #pragma unroll
   for (int s = 0; s < NS; ++s)
     k7[s] = f[s] + ((R::C71 * idt) * k1[s] + (R::C72 * idt) * k2[s] + (R::C73 * idt) * k3[s] +
                     (R::C74 * idt) * k4[s] + (R::C75 * idt) * k5[s] + (R::C76 * idt) * k6[s]);
-  wsolve<NS>(m, k7);
+  wsolve_(k7);
   st(6, k7);
   // stage 8
 #pragma unroll
   for (int s = 0; s < NS; ++s) u[s] = u[s] + k7[s];
-  eval_rhs<NS, double>(cx, m.rd, t0 + dt, u, f);
+  eval_rhs_(t0 + dt, u, f);
 #pragma unroll
   for (int s = 0; s < NS; ++s)
     err[s] = f[s] + ((R::C81 * idt) * k1[s] + (R::C82 * idt) * k2[s] + (R::C83 * idt) * k3[s] +
                      (R::C84 * idt) * k4[s] + (R::C85 * idt) * k5[s] + (R::C86 * idt) * k6[s] +
                      (R::C87 * idt) * k7[s]);
-  wsolve<NS>(m, err);
+  wsolve_(err);
   st(7, err);
 #pragma unroll
   for (int s = 0; s < NS; ++s) y1[s] = u[s] + err[s];
