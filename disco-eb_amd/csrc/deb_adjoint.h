@@ -577,55 +577,68 @@ __device__ __forceinline__ void row_feature_adj(uint32_t d, const Ctx &cx, const
 }
 
 // First-order contraction of sum_rows w_row f_row(a, Y) along (a, theta):
-//   feature_partials per lane, the partials of its rows' sum_fields mul * coef
-//                    with respect to the features, accumulated into fb (plain
-//                    doubles, no cross-lane sums), for K contractions in one
-//                    pass over the rows;
-//   lane_dot         after fb is summed over the wave, sum_m dF_m fb_m for the
-//                    lane's own tangent direction (lane_ctx).
+//   row_muls           the multiplier of every coefficient field of every row;
+//   feature_partials_m per lane, the partials of its rows' sum_fields
+//                      mul * coef with respect to the features, accumulated
+//                      into fb (plain doubles, no cross-lane sums; linear in
+//                      mul, so multipliers of contractions at the same
+//                      features may be summed first);
+//   lane_dot           after fb is summed over the wave, sum_m dF_m fb_m for
+//                      the lane's own tangent direction (lane_ctx).
 // The functionals (h', eta', Pi, f_1) enter through their double values in rs,
 // which is exact at first order: d(u H) = du H + u dH with dH = sum_rows dc_h Y,
 // so sum_rows w u dH = s_h sum_rows dc_h Y (multiplier Y s_h of the c_h field).
-template <int NS, int K>
-DEB_OUTLINE void feature_partials(const Ctx &cx, const uint32_t (&rd)[NS], const double (&fv)[K][FT_N],
-                                  const double (&Y)[K][NS], const double (&w)[K][NS], const RowSums (&rs)[K],
-                                  double (&fb)[K][FT_N]) {
+
+// multipliers of the coefficient fields of every row in sum_rows w f_row(Y)
+// (the contraction's per-row weights; rs: the functionals' values)
+template <int NS>
+__device__ __forceinline__ void row_muls(const double (&Y)[NS], const double (&w)[NS], const RowSums &rs,
+                                         double (&mul)[NS][11]) {
   const int lane = lane_id();
-  double ydg[K], ytg[K], wahp[K];
+  const double ydg = readlane(Y[0], L_DG), ytg = readlane(Y[0], L_TG);
+  const double wahp = readlane(w[0], L_AHP);
 #pragma unroll
-  for (int q = 0; q < K; ++q) {
-    ydg[q] = readlane(Y[q][0], L_DG);
-    ytg[q] = readlane(Y[q][0], L_TG);
-    wahp[q] = readlane(w[q][0], L_AHP);
+  for (int s = 0; s < NS; ++s) {
+    const double ys = Y[s], ws = w[s];
+    const double ym = from_prev(ys), yp = from_next(ys);
+    const double lp = (s == 0) ? (lane == L_DG ? ytg : (lane == L_TG ? ydg : 0.0)) : 0.0;
+    mul[s][C_SUB] = ws * ym;
+    mul[s][C_DIAG] = ws * ys;
+    mul[s][C_SUP] = ws * yp;
+    mul[s][C_UH] = ws * rs.H;
+    mul[s][C_UETA] = ws * rs.E;
+    mul[s][C_UPI] = ws * rs.Pi;
+    mul[s][C_CH] = ys * rs.sh;
+    mul[s][C_CETA] = ys * rs.se;
+    mul[s][C_CPI] = ys * rs.sp;
+    mul[s][C_C1] = ys * wahp;
+    // the leaf field is unused by the a-row: there it carries the row weight w_a
+    mul[s][C_LEAF] = (s == 0 && lane == L_A) ? ws : ws * lp;
   }
+}
+
+// fb[q] += partials of sum_rows sum_fields mul[q][row][field] coef_field with
+// respect to the features fv[q] (K contractions in one pass over the rows)
+template <int NS, int K>
+DEB_OUTLINE void feature_partials_m(const Ctx &cx, const uint32_t (&rd)[NS], const double (&fv)[K][FT_N],
+                                    const double (&mul)[K][NS][11], double (&fb)[K][FT_N]) {
+  const int lane = lane_id();
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
     const uint32_t ty = rd_type(rd[s]);
     const bool qrow = ty >= R_Q0 && ty <= R_QLMAX;
     const double iq = qrow ? 1.0 / cx.q[rd_q(rd[s])] : 0.0;
-    double mul[K][11], vq[K], xq[K], vb[K];
+    double ms[K][11], vq[K], xq[K], vb[K];
 #pragma unroll
     for (int q = 0; q < K; ++q) {
-      const double ys = Y[q][s], ws = w[q][s];
-      const double ym = from_prev(ys), yp = from_next(ys);
-      const double lp = (s == 0) ? (lane == L_DG ? ytg[q] : (lane == L_TG ? ydg[q] : 0.0)) : 0.0;
-      mul[q][C_SUB] = ws * ym;
-      mul[q][C_DIAG] = ws * ys;
-      mul[q][C_SUP] = ws * yp;
-      mul[q][C_UH] = ws * rs[q].H;
-      mul[q][C_UETA] = ws * rs[q].E;
-      mul[q][C_UPI] = ws * rs[q].Pi;
-      mul[q][C_CH] = ys * rs[q].sh;
-      mul[q][C_CETA] = ys * rs[q].se;
-      mul[q][C_CPI] = ys * rs[q].sp;
-      mul[q][C_C1] = ys * wahp[q];
-      mul[q][C_LEAF] = ws * lp;
+#pragma unroll
+      for (int f = 0; f < 11; ++f) ms[q][f] = mul[q][s][f];
       // v_q(x), x = a amnu / q
       xq[q] = fv[q][FT_A] * fv[q][FT_AMNU] * iq;
       vq[q] = 1.0 / sqrt(1.0 + xq[q] * xq[q]);
       vb[q] = 0.0;
     }
-    row_feature_adj<K>(rd[s], cx, fv, vq, mul, fb, vb);
+    row_feature_adj<K>(rd[s], cx, fv, vq, ms, fb, vb);
     if (qrow) {
 #pragma unroll
       for (int q = 0; q < K; ++q) {
@@ -635,11 +648,11 @@ DEB_OUTLINE void feature_partials(const Ctx &cx, const uint32_t (&rd)[NS], const
       }
     }
   }
-  if (lane == L_A) {  // f_a = aH a
+  if (lane == L_A) {  // f_a = aH a, weight w_a (row_muls)
 #pragma unroll
     for (int q = 0; q < K; ++q) {
-      fb[q][FT_AH] += w[q][0] * fv[q][FT_A];
-      fb[q][FT_A] += w[q][0] * fv[q][FT_AH];
+      fb[q][FT_AH] += mul[q][0][C_LEAF] * fv[q][FT_A];
+      fb[q][FT_A] += mul[q][0][C_LEAF] * fv[q][FT_AH];
     }
   }
 }
@@ -784,14 +797,15 @@ __device__ __forceinline__ void rodas5_step_vjp(Mode<NS> &m, Ctx &cx, double t0,
 #pragma unroll
   for (int j = 0; j < MAXDIR; ++j) th_acc[j] = 0.0;
   double a0b = 0.0, t0b = 0.0, hb = 0.0;
-  // Feature partials of each stage, in one pass over the rows per stage:
-  // set 0 for f(t_i, u_i) (features at a_i), set 1 for the J(t0, y0) k_i term,
-  // whose features at a0 are shared by all stages: its partials are summed
-  // over the stages and contracted once after the loop.
-  double fvk[2][FT_N], fbk[2][FT_N];
-  feature_values(cx, a0, fvk[1]);
+  // Feature partials: per stage for f(t_i, u_i) (features at a_i); the
+  // J(t0, y0) k_i terms share the features at a0 and are linear in their row
+  // multipliers, which are summed over the stages and passed over once.
+  double fvJ[1][FT_N], mulJ[1][NS][11];
+  feature_values(cx, a0, fvJ[0]);
 #pragma unroll
-  for (int q = 0; q < FT_N; ++q) fbk[1][q] = 0.0;
+  for (int q = 0; q < NS; ++q)
+#pragma unroll
+    for (int f = 0; f < 11; ++f) mulJ[0][q][f] = 0.0;
 #pragma nounroll
   for (int i = 7; i >= 0; --i) {
     double rb[NS], ki[NS];
@@ -841,22 +855,24 @@ __device__ __forceinline__ void rodas5_step_vjp(Mode<NS> &m, Ctx &cx, double t0,
     double ub[NS];
     {
       const double ai = readlane(u[0], L_A);
-      double Yk[2][NS], wk[2][NS];
-      RowSums rsk[2];
-      rsk[0] = coef_pass<NS, true>(cx, m.rd, ai, iti, rb, u, ub, fvk[0]);
+      double fv[1][FT_N], fb[1][FT_N], mul[1][NS][11];
+      const RowSums rs = coef_pass<NS, true>(cx, m.rd, ai, iti, rb, u, ub, fv[0]);
+      row_muls<NS>(u, rb, rs, mul[0]);
 #pragma unroll
-      for (int s = 0; s < NS; ++s) {
-        Yk[0][s] = u[s];
-        wk[0][s] = rb[s];
-        Yk[1][s] = ki[s];
-        wk[1][s] = (s == 0 && lane == L_A) ? 0.0 : rb[s];
+      for (int q = 0; q < FT_N; ++q) fb[0][q] = 0.0;
+      feature_partials_m<NS, 1>(cx, m.rd, fv, mul, fb);
+      {
+        double wr[NS], mr[NS][11];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) wr[s] = (s == 0 && lane == L_A) ? 0.0 : rb[s];
+        row_muls<NS>(ki, wr, jac_sums<NS>(m, ki, wr), mr);
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+#pragma unroll
+          for (int f = 0; f < 11; ++f) mulJ[0][s][f] += mr[s][f];
       }
-      rsk[1] = jac_sums<NS>(m, Yk[1], wk[1]);
-#pragma unroll
-      for (int q = 0; q < FT_N; ++q) fbk[0][q] = 0.0;
-      feature_partials<NS, 2>(cx, m.rd, fvk, Yk, wk, rsk, fbk);
-      wave_sum<FT_N>(fbk[0]);
-      const double g = lane_contract(cx, ai, fbk[0]);
+      wave_sum<FT_N>(fb[0]);
+      const double g = lane_contract(cx, ai, fb[0]);
       const double ga = readlane(g, 0);
       if (lane == L_A) ub[0] += ga;
       double tl[MAXDIR];
@@ -877,8 +893,12 @@ __device__ __forceinline__ void rodas5_step_vjp(Mode<NS> &m, Ctx &cx, double t0,
   }
   // the J(t0, y0) k_i terms, contracted with the feature tangents at a0
   {
-    wave_sum<FT_N>(fbk[1]);
-    const double g = lane_contract(cx, a0, fbk[1]);
+    double fbJ[1][FT_N];
+#pragma unroll
+    for (int q = 0; q < FT_N; ++q) fbJ[0][q] = 0.0;
+    feature_partials_m<NS, 1>(cx, m.rd, fvJ, mulJ, fbJ);
+    wave_sum<FT_N>(fbJ[0]);
+    const double g = lane_contract(cx, a0, fbJ[0]);
     a0b += readlane(g, 0);
     double tl[MAXDIR];
     lane_gather(g, cx.ndir, tl);
