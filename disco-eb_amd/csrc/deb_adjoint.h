@@ -199,11 +199,36 @@ DEB_OUTLINE void wsolveT(const Mode<NS> &m, double (&x)[NS]) {
 // sum over all rows of w_row f_row(a, Y): f is model_synchronous at scale
 // factor a (a T with the caller's seeds) for the state Y; exact in T to any
 // order (the functionals are carried in T).
+// one row's share of contract: acc[0] += w (local stencil part), acc[1..3] +=
+// w u{h,eta,pi}, acc[4..7] += c{h,eta,pi,1} y; lp: the leaf partner's value on
+// the delta_gamma / theta_gamma lanes (else 0); arow: the a-row (f_a = aH a).
+// A call per row keeps the nested-dual coefficient arithmetic out of the
+// calling kernel's registers.
+template <class T>
+DEB_OUTLINE void contract_row(Ctx &cx, uint32_t d, const Bg<T> &b, double invtau, double ym, double y, double yp,
+                              double lp, bool arow, double w, T (&acc)[8]) {
+  const Coef<T> r = row_coef<T>(d, b, cx, invtau);
+  T loc = r.sub * ym + r.diag * y + r.sup * yp + r.leaf * lp;
+  if (arow) loc = b.aH * b.a;
+  acc[0] = acc[0] + loc * w;
+  acc[1] = acc[1] + r.uh * w;
+  acc[2] = acc[2] + r.ueta * w;
+  acc[3] = acc[3] + r.upi * w;
+  acc[4] = acc[4] + r.ch * y;
+  acc[5] = acc[5] + r.ceta * y;
+  acc[6] = acc[6] + r.cpi * y;
+  acc[7] = acc[7] + r.c1 * y;
+}
+
+template <class T>
+DEB_OUTLINE void background_call(Ctx &cx, T a, Bg<T> &b) { b = background<T>(cx, a); }
+
 template <int NS, class T>
 __device__ __forceinline__ T contract(Ctx &cx, const uint32_t (&rd)[NS], T a, double invtau, const double (&Y)[NS],
                                       const double (&w)[NS]) {
   const int lane = lane_id();
-  Bg<T> b = background<T>(cx, a);
+  Bg<T> b;
+  background_call<T>(cx, a, b);
   // [0] local stencil part, [1..3] sum w*u{h,eta,pi}, [4..7] sum c{h,eta,pi,1}*Y
   T acc[8];
 #pragma unroll
@@ -211,22 +236,9 @@ __device__ __forceinline__ T contract(Ctx &cx, const uint32_t (&rd)[NS], T a, do
   const double ydg = readlane(Y[0], L_DG), ytg = readlane(Y[0], L_TG);
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
-    Coef<T> r = row_coef<T>(rd[s], b, cx, invtau);
     const double ym = from_prev(Y[s]), yp = from_next(Y[s]);
-    T loc = r.sub * ym + r.diag * Y[s] + r.sup * yp;
-    if (s == 0) {
-      if (lane == L_DG) loc = loc + r.leaf * ytg;
-      if (lane == L_TG) loc = loc + r.leaf * ydg;
-      if (lane == L_A) loc = b.aH * b.a;
-    }
-    acc[0] = acc[0] + loc * w[s];
-    acc[1] = acc[1] + r.uh * w[s];
-    acc[2] = acc[2] + r.ueta * w[s];
-    acc[3] = acc[3] + r.upi * w[s];
-    acc[4] = acc[4] + r.ch * Y[s];
-    acc[5] = acc[5] + r.ceta * Y[s];
-    acc[6] = acc[6] + r.cpi * Y[s];
-    acc[7] = acc[7] + r.c1 * Y[s];
+    const double lp = (s == 0) ? (lane == L_DG ? ytg : (lane == L_TG ? ydg : 0.0)) : 0.0;
+    contract_row<T>(cx, rd[s], b, invtau, ym, Y[s], yp, lp, s == 0 && lane == L_A, w[s], acc);
   }
   t_wave_sum<T, 8>(acc);
   const double wahp = readlane(w[0], L_AHP);
