@@ -23,7 +23,6 @@ namespace deb {
 
 constexpr int MAXDIR = DEB_MAXDIR;
 using TD = dn<1 + MAXDIR>;          // (a or tau, theta_1..theta_MAXDIR)
-using TH = dn<1 + MAXDIR, dual>;    // the same, each component also differentiated along a
 
 // The row passes of the reverse step are real calls rather than inlined:
 // inlined into the step they leave one kernel body with thousands of spilled
@@ -196,57 +195,9 @@ DEB_OUTLINE void wsolveT(const Mode<NS> &m, double (&x)[NS]) {
 // ---------------------------------------------------------------------------
 // right-hand side contractions
 
-// sum over all rows of w_row f_row(a, Y): f is model_synchronous at scale
-// factor a (a T with the caller's seeds) for the state Y; exact in T to any
-// order (the functionals are carried in T).
-// one row's share of contract: acc[0] += w (local stencil part), acc[1..3] +=
-// w u{h,eta,pi}, acc[4..7] += c{h,eta,pi,1} y; lp: the leaf partner's value on
-// the delta_gamma / theta_gamma lanes (else 0); arow: the a-row (f_a = aH a).
-// A call per row keeps the nested-dual coefficient arithmetic out of the
-// calling kernel's registers.
-template <class T>
-DEB_OUTLINE void contract_row(Ctx &cx, uint32_t d, const Bg<T> &b, double invtau, double ym, double y, double yp,
-                              double lp, bool arow, double w, T (&acc)[8]) {
-  const Coef<T> r = row_coef<T>(d, b, cx, invtau);
-  T loc = r.sub * ym + r.diag * y + r.sup * yp + r.leaf * lp;
-  if (arow) loc = b.aH * b.a;
-  acc[0] = acc[0] + loc * w;
-  acc[1] = acc[1] + r.uh * w;
-  acc[2] = acc[2] + r.ueta * w;
-  acc[3] = acc[3] + r.upi * w;
-  acc[4] = acc[4] + r.ch * y;
-  acc[5] = acc[5] + r.ceta * y;
-  acc[6] = acc[6] + r.cpi * y;
-  acc[7] = acc[7] + r.c1 * y;
-}
-
-template <class T>
-DEB_OUTLINE void background_call(Ctx &cx, T a, Bg<T> &b) { b = background<T>(cx, a); }
-
-template <int NS, class T>
-__device__ __forceinline__ T contract(Ctx &cx, const uint32_t (&rd)[NS], T a, double invtau, const double (&Y)[NS],
-                                      const double (&w)[NS]) {
-  const int lane = lane_id();
-  Bg<T> b;
-  background_call<T>(cx, a, b);
-  // [0] local stencil part, [1..3] sum w*u{h,eta,pi}, [4..7] sum c{h,eta,pi,1}*Y
-  T acc[8];
-#pragma unroll
-  for (int i = 0; i < 8; ++i) acc[i] = tconst<T>(0.0);
-  const double ydg = readlane(Y[0], L_DG), ytg = readlane(Y[0], L_TG);
-#pragma unroll
-  for (int s = 0; s < NS; ++s) {
-    const double ym = from_prev(Y[s]), yp = from_next(Y[s]);
-    const double lp = (s == 0) ? (lane == L_DG ? ytg : (lane == L_TG ? ydg : 0.0)) : 0.0;
-    contract_row<T>(cx, rd[s], b, invtau, ym, Y[s], yp, lp, s == 0 && lane == L_A, w[s], acc);
-  }
-  t_wave_sum<T, 8>(acc);
-  const double wahp = readlane(w[0], L_AHP);
-  return acc[0] + acc[1] * acc[4] + acc[2] * acc[5] + acc[3] * acc[6] + acc[7] * wahp;
-}
-
 // out = A(a)^T w for the y-linear part of f (rows other than a; the a-column
-// and the a-row are handled by contract).  T = dual gives (A^T w, A'(a)^T w).
+// and the a-row are handled through the features).  T = dual gives
+// (A^T w, A'(a)^T w).
 template <int NS, class T>
 DEB_OUTLINE void tapply(Ctx &cx, const uint32_t (&rd)[NS], T a, double invtau, const double (&w)[NS],
                                        T (&out)[NS]) {
@@ -333,10 +284,12 @@ __device__ __forceinline__ void features_t(Ctx &cx, T a, T (&F)[FT_N]) {
 // Values of the functionals of Y and of the "where they enter" sums of w that
 // the first-order contraction below needs, plus (optionally) the transposed
 // right-hand side A(a)^T w -- one plain-double pass over the rows.
-struct RowSums {
-  double H, E, Pi, F1;  // sum c_{h,eta,pi,1} Y
-  double sh, se, sp;    // sum u_{h,eta,pi} w
+template <class T>
+struct RowSumsT {
+  T H, E, Pi, F1;  // sum c_{h,eta,pi,1} Y
+  T sh, se, sp;    // sum u_{h,eta,pi} w
 };
+using RowSums = RowSumsT<double>;
 
 template <int NS, bool TAPPLY>
 DEB_OUTLINE RowSums coef_pass(Ctx &cx, const uint32_t (&rd)[NS], double a, double invtau,
@@ -384,10 +337,10 @@ DEB_OUTLINE RowSums coef_pass(Ctx &cx, const uint32_t (&rd)[NS], double a, doubl
 // features (accumulated into fb[q]) and to its bin velocity (vb[q]), for K
 // contractions at once (one pass of the divergent row switch); fv[q]: the
 // feature values of contraction q
-template <int K>
-__device__ __forceinline__ void row_feature_adj(uint32_t d, const Ctx &cx, const double (&fv)[K][FT_N],
-                                                const double (&vq)[K], const double (&mul)[K][11],
-                                                double (&fb)[K][FT_N], double (&vb)[K]) {
+template <int K, class T = double>
+__device__ __forceinline__ void row_feature_adj(uint32_t d, const Ctx &cx, const T (&fv)[K][FT_N],
+                                                const T (&vq)[K], const T (&mul)[K][11], T (&fb)[K][FT_N],
+                                                T (&vb)[K]) {
   const double k = cx.k, k2 = k * k;
   const uint32_t t = rd_type(d);
   const double l = (double)rd_l(d);
@@ -491,8 +444,9 @@ __device__ __forceinline__ void row_feature_adj(uint32_t d, const Ctx &cx, const
     case R_Q0: {
 #pragma unroll
       for (int q = 0; q < K; ++q) {
-        const double wq = cx.w[qi], RN = fv[q][FT_RN];
-        const double iv = 1.0 / vq[q];
+        const double wq = cx.w[qi];
+        const T RN = fv[q][FT_RN];
+        const T iv = 1.0 / vq[q];
         vb[q] += -k * mul[q][C_SUP];
         // ch = RN wq / v * iaH
         fb[q][FT_RN] += mul[q][C_CH] * wq * iv * fv[q][FT_IAH];
@@ -534,8 +488,8 @@ __device__ __forceinline__ void row_feature_adj(uint32_t d, const Ctx &cx, const
 #pragma unroll
       for (int q = 0; q < K; ++q) {
         if (!cx.lambda) {
-          const double w = fv[q][FT_W], ca2 = fv[q][FT_CA2], cs2Q = fv[q][FT_CS2Q], RQ = fv[q][FT_RQ];
-          const double opw = 1.0 + w, g = fv[q][FT_AH] * fv[q][FT_AH] / k2;
+          const T w = fv[q][FT_W], ca2 = fv[q][FT_CA2], cs2Q = fv[q][FT_CS2Q], RQ = fv[q][FT_RQ];
+          const T opw = 1.0 + w, g = fv[q][FT_AH] * fv[q][FT_AH] / k2;
           // diag = -3 (cs2Q - w) aH
           fb[q][FT_CS2Q] += -3.0 * fv[q][FT_AH] * mul[q][C_DIAG];
           fb[q][FT_W] += 3.0 * fv[q][FT_AH] * mul[q][C_DIAG];
@@ -561,8 +515,8 @@ __device__ __forceinline__ void row_feature_adj(uint32_t d, const Ctx &cx, const
 #pragma unroll
       for (int q = 0; q < K; ++q) {
         if (!cx.lambda) {
-          const double w = fv[q][FT_W], ca2 = fv[q][FT_CA2], cs2Q = fv[q][FT_CS2Q], RQ = fv[q][FT_RQ];
-          const double opw = 1.0 + w;
+          const T w = fv[q][FT_W], ca2 = fv[q][FT_CA2], cs2Q = fv[q][FT_CS2Q], RQ = fv[q][FT_RQ];
+          const T opw = 1.0 + w;
           // sub = cs2Q / opw k2
           fb[q][FT_CS2Q] += mul[q][C_SUB] * k2 / opw;
           fb[q][FT_W] += -mul[q][C_SUB] * cs2Q * k2 / (opw * opw);
@@ -603,9 +557,9 @@ __device__ __forceinline__ void row_feature_adj(uint32_t d, const Ctx &cx, const
 
 // multipliers of the coefficient fields of every row in sum_rows w f_row(Y)
 // (the contraction's per-row weights; rs: the functionals' values)
-template <int NS>
-__device__ __forceinline__ void row_muls(const double (&Y)[NS], const double (&w)[NS], const RowSums &rs,
-                                         double (&mul)[NS][11]) {
+template <int NS, class T = double>
+__device__ __forceinline__ void row_muls(const double (&Y)[NS], const double (&w)[NS], const RowSumsT<T> &rs,
+                                         T (&mul)[NS][11]) {
   const int lane = lane_id();
   const double ydg = readlane(Y[0], L_DG), ytg = readlane(Y[0], L_TG);
   const double wahp = readlane(w[0], L_AHP);
@@ -614,47 +568,47 @@ __device__ __forceinline__ void row_muls(const double (&Y)[NS], const double (&w
     const double ys = Y[s], ws = w[s];
     const double ym = from_prev(ys), yp = from_next(ys);
     const double lp = (s == 0) ? (lane == L_DG ? ytg : (lane == L_TG ? ydg : 0.0)) : 0.0;
-    mul[s][C_SUB] = ws * ym;
-    mul[s][C_DIAG] = ws * ys;
-    mul[s][C_SUP] = ws * yp;
+    mul[s][C_SUB] = tconst<T>(ws * ym);
+    mul[s][C_DIAG] = tconst<T>(ws * ys);
+    mul[s][C_SUP] = tconst<T>(ws * yp);
     mul[s][C_UH] = ws * rs.H;
     mul[s][C_UETA] = ws * rs.E;
     mul[s][C_UPI] = ws * rs.Pi;
     mul[s][C_CH] = ys * rs.sh;
     mul[s][C_CETA] = ys * rs.se;
     mul[s][C_CPI] = ys * rs.sp;
-    mul[s][C_C1] = ys * wahp;
+    mul[s][C_C1] = tconst<T>(ys * wahp);
     // the leaf field is unused by the a-row: there it carries the row weight w_a
-    mul[s][C_LEAF] = (s == 0 && lane == L_A) ? ws : ws * lp;
+    mul[s][C_LEAF] = tconst<T>((s == 0 && lane == L_A) ? ws : ws * lp);
   }
 }
 
 // fb[q] += partials of sum_rows sum_fields mul[q][row][field] coef_field with
 // respect to the features fv[q] (K contractions in one pass over the rows)
-template <int NS, int K>
-DEB_OUTLINE void feature_partials_m(const Ctx &cx, const uint32_t (&rd)[NS], const double (&fv)[K][FT_N],
-                                    const double (&mul)[K][NS][11], double (&fb)[K][FT_N]) {
+template <int NS, int K, class T = double>
+DEB_OUTLINE void feature_partials_m(const Ctx &cx, const uint32_t (&rd)[NS], const T (&fv)[K][FT_N],
+                                    const T (&mul)[K][NS][11], T (&fb)[K][FT_N]) {
   const int lane = lane_id();
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
     const uint32_t ty = rd_type(rd[s]);
     const bool qrow = ty >= R_Q0 && ty <= R_QLMAX;
     const double iq = qrow ? 1.0 / cx.q[rd_q(rd[s])] : 0.0;
-    double ms[K][11], vq[K], xq[K], vb[K];
+    T ms[K][11], vq[K], xq[K], vb[K];
 #pragma unroll
     for (int q = 0; q < K; ++q) {
 #pragma unroll
       for (int f = 0; f < 11; ++f) ms[q][f] = mul[q][s][f];
       // v_q(x), x = a amnu / q
       xq[q] = fv[q][FT_A] * fv[q][FT_AMNU] * iq;
-      vq[q] = 1.0 / sqrt(1.0 + xq[q] * xq[q]);
-      vb[q] = 0.0;
+      vq[q] = 1.0 / dsqrt(1.0 + xq[q] * xq[q]);
+      vb[q] = tconst<T>(0.0);
     }
-    row_feature_adj<K>(rd[s], cx, fv, vq, ms, fb, vb);
+    row_feature_adj<K, T>(rd[s], cx, fv, vq, ms, fb, vb);
     if (qrow) {
 #pragma unroll
       for (int q = 0; q < K; ++q) {
-        const double dvdx = -xq[q] * vq[q] * vq[q] * vq[q];
+        const T dvdx = -xq[q] * vq[q] * vq[q] * vq[q];
         fb[q][FT_A] += vb[q] * dvdx * fv[q][FT_AMNU] * iq;
         fb[q][FT_AMNU] += vb[q] * dvdx * fv[q][FT_A] * iq;
       }
@@ -711,6 +665,54 @@ DEB_OUTLINE double lane_contract(const Ctx &cx, double a, const double (&fbs)[FT
   TL F[FT_N];
   features_t<TL>(cl, seed_tl(a), F);
   return lane_dot(F, fbs);
+}
+
+template <class T>
+DEB_OUTLINE void features_call(Ctx &cx, T a, T (&F)[FT_N]) { features_t<T>(cx, a, F); }
+
+// the functional sums of sum_rows w f_row(a, Y) in T (for T = dual along a:
+// their values and a-derivatives)
+template <int NS, class T>
+DEB_OUTLINE void row_sums_t(Ctx &cx, const uint32_t (&rd)[NS], T a, double invtau, const double (&w)[NS],
+                            const double (&Y)[NS], RowSumsT<T> &rs) {
+  const Bg<T> b = background<T>(cx, a);
+  T part[7];
+#pragma unroll
+  for (int i = 0; i < 7; ++i) part[i] = tconst<T>(0.0);
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    const Coef<T> r = row_coef<T>(rd[s], b, cx, invtau);
+    part[0] = part[0] + r.ch * Y[s];
+    part[1] = part[1] + r.ceta * Y[s];
+    part[2] = part[2] + r.cpi * Y[s];
+    part[3] = part[3] + r.c1 * Y[s];
+    part[4] = part[4] + r.uh * w[s];
+    part[5] = part[5] + r.ueta * w[s];
+    part[6] = part[6] + r.upi * w[s];
+  }
+  t_wave_sum<T, 7>(part);
+  rs = RowSumsT<T>{part[0], part[1], part[2], part[3], part[4], part[5], part[6]};
+}
+
+// Second order along one direction per lane: with P_m = dS/dF_m and
+// Q_m = dP_m/da (wave-uniform) for S(F(a, theta)), returns on lane direction c
+// d/dc (dS/da) = sum_m Q_m dF_m/dc + P_m d2F_m/(da dc)  (c = a on lane & 7 == 0).
+// Features in nested duals: outer component along the lane's direction, inner
+// along a.
+using TL2 = dn<2, dual>;
+DEB_OUTLINE double lane_contract2(const Ctx &cx, double a, const double (&P)[FT_N], const double (&Q)[FT_N]) {
+  Ctx cl = lane_ctx(cx);
+  TL2 as;
+  as = a;
+  as.v.d = 1.0;
+  as.d[0].v = (lane_id() & 7) == 0 ? 1.0 : 0.0;
+  TL2 F[FT_N];
+  features_t<TL2>(cl, as, F);
+  double g = 0.0;
+#pragma unroll
+  for (int m = 0; m < FT_N; ++m)
+    g += Q[m] * (F[m].d[0].v + F[m].d[1].v) + P[m] * (F[m].d[0].d + F[m].d[1].d);
+  return g;
 }
 
 // the lane-direction result of every direction: out[j] = d/dtheta_j
@@ -926,14 +928,35 @@ __device__ __forceinline__ void rodas5_step_vjp(Mode<NS> &m, Ctx &cx, double t0,
     tapply<NS, dual>(cx, m.rd, ad, it0, rho, gy);
 #pragma unroll
     for (int s = 0; s < NS; ++s) y0b[s] += gy[s].d;
-    TH ah;
-    ah = a0;
-    ah.v.d = 1.0;
-    ah.d[0].v = 1.0;
-    TH g = contract<NS, TH>(cx, m.rd, ah, it0, y0, rho);
-    a0b += g.d[0].d;
+    // S = rho^T f(a, y0): feature partials P and their a-derivative Q in one
+    // pass in dual arithmetic along a, then the features' second derivatives
+    dual fvd[1][FT_N], fbd[1][FT_N], muld[1][NS][11];
+    features_call<dual>(cx, ad, fvd[0]);
+    RowSumsT<dual> rsd;
+    row_sums_t<NS, dual>(cx, m.rd, ad, it0, rho, y0, rsd);
+    row_muls<NS, dual>(y0, rho, rsd, muld[0]);
 #pragma unroll
-    for (int j = 0; j < MAXDIR; ++j) th_acc[j] += g.d[j + 1].d;
+    for (int q = 0; q < FT_N; ++q) fbd[0][q] = 0.0;
+    feature_partials_m<NS, 1, dual>(cx, m.rd, fvd, muld, fbd);
+    double PQ[2 * FT_N];
+#pragma unroll
+    for (int q = 0; q < FT_N; ++q) {
+      PQ[q] = fbd[0][q].v;
+      PQ[FT_N + q] = fbd[0][q].d;
+    }
+    wave_sum<2 * FT_N>(PQ);
+    double P[FT_N], Q[FT_N];
+#pragma unroll
+    for (int q = 0; q < FT_N; ++q) {
+      P[q] = PQ[q];
+      Q[q] = PQ[FT_N + q];
+    }
+    const double g = lane_contract2(cx, a0, P, Q);
+    a0b += readlane(g, 0);
+    double tl[MAXDIR];
+    lane_gather(g, cx.ndir, tl);
+#pragma unroll
+    for (int j = 0; j < MAXDIR; ++j) th_acc[j] += tl[j];
   }
   // dT = D(t0) y0
   {

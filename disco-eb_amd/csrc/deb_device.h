@@ -79,6 +79,11 @@ struct dual {
   }
 };
 __device__ __forceinline__ dual mkd(double v, double d = 0.0) { return {v, d}; }
+__device__ __forceinline__ dual &operator+=(dual &a, dual b) {
+  a.v += b.v;
+  a.d += b.d;
+  return a;
+}
 __device__ __forceinline__ dual operator+(dual a, dual b) { return {a.v + b.v, a.d + b.d}; }
 __device__ __forceinline__ dual operator-(dual a, dual b) { return {a.v - b.v, a.d - b.d}; }
 __device__ __forceinline__ dual operator-(dual a) { return {-a.v, -a.d}; }
