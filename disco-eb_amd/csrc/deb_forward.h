@@ -371,6 +371,136 @@ __device__ __forceinline__ void integrate_mode(const Params<NS> &P, int ci, int 
   }
 }
 
+// evolve_modes_batched (perturbations.py:820-922) for one group of B
+// consecutive k (kb .. kb + B - 1) of cosmology ci.  One step sequence serves
+// the group: its start time is the minimum of the modes' start times
+// (determine_start_times_per_batch :786-803) and the controller sees the RMS
+// over all modes' filtered, weighted errors (rms_norm_filtered_batched
+// :690-694).  The wavefront steps the modes one after the other; each mode's
+// state and proposal live in ws (2 * NS * WAVE doubles per mode, lane-owned).
+template <int NS>
+__device__ __forceinline__ void integrate_batch(const Params<NS> &P, int ci, int kb, int B, double *lds,
+                                                double *ws) {
+  const int lane = lane_id();
+  const int64_t mode0 = (int64_t)ci * P.nk + kb;
+  Ctx cx;
+  make_ctx<NS>(cx, P, ci, P.kmodes[mode0]);
+  Mode<NS> m;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) m.rd[s] = P.rowdesc[s][lane];
+  m.nlev = mode_levels(P.cfg, P.nq);
+  auto state = [&](int b, int which) -> double * {
+    return ws + ((mode0 + b) * 2 + which) * (int64_t)(NS * WAVE) + lane;
+  };
+  const double *tout = P.tau_out + (int64_t)ci * P.nout;
+  double tmin = tout[0], tmax = tout[0];
+  for (int i = 1; i < P.nout; ++i) {
+    tmin = fmin(tmin, tout[i]);
+    tmax = fmax(tmax, tout[i]);
+  }
+  double t0 = start_time(cx, tmin);
+  for (int b = 1; b < B; ++b) {
+    cx.k = P.kmodes[mode0 + b];
+    t0 = fmin(t0, start_time(cx, tmin));
+  }
+  const double t1 = tmax;
+  for (int b = 0; b < B; ++b) {
+    cx.k = P.kmodes[mode0 + b];
+    double y[NS];
+    initial_conditions<NS>(cx, m.rd, t0, y);
+    double *dst = state(b, 0);
+#pragma unroll
+    for (int s = 0; s < NS; ++s) dst[s * WAVE] = y[s];
+  }
+  const deb_config_t &cf = P.cfg;
+  const double eo = 5.0;
+  const double coeff1 = (cf.icoeff + cf.pcoeff + cf.dcoeff) / eo;
+  const double coeff2 = -(cf.pcoeff + 2.0 * cf.dcoeff) / eo;
+  const double coeff3 = cf.dcoeff / eo;
+  double tprev = t0;
+  double tnext = fmin(t0 + fmin(t0 / 4.0, 0.5 * (t1 - t0)), t1);
+  double prev_inv = 1.0, prev_prev_inv = 1.0;
+  int iout = 0, steps = 0;
+  int status = DEB_STATUS_OK;
+  const int width = cf.return_full ? P.n : DEB_NOUT_FIELDS;
+  while (tprev < t1) {
+    if (steps >= cf.max_steps) {
+      status = DEB_STATUS_MAX_STEPS;
+      break;
+    }
+    const double dt = tnext - tprev;
+    double e2 = 0.0;
+    for (int b = 0; b < B; ++b) {
+      const double k = P.kmodes[mode0 + b];
+      cx.k = k;
+      double y[NS], y1[NS], err[NS];
+      const double *src = state(b, 0);
+#pragma unroll
+      for (int s = 0; s < NS; ++s) y[s] = src[s * WAVE];
+      rodas5_step<NS>(m, cx, tprev, dt, y, y1, err);
+      // error-norm weights: indices (a, eta, delta_c, delta_b, theta_b, delta_gamma) of slot 0
+      double ew = 0.0;
+      if (lane == L_A || lane == L_DC || lane == L_DB || lane == L_DG) ew = 1.0;
+      if (lane == L_ETA) ew = k * k;
+      if (lane == L_TB) ew = 1.0 / (k * k);
+      const double sc = err[0] / (cf.atol + fmax(fabs(y[0]), fabs(y1[0])) * cf.rtol);
+      const double x = sc * ew;
+      e2 += wave_sum1(x * x);
+      double *dst = state(b, 1);
+#pragma unroll
+      for (int s = 0; s < NS; ++s) dst[s * WAVE] = y1[s];
+    }
+    const double nrm = sqrt(e2 / (6.0 * B));
+    if (!isfinite(nrm)) {
+      status = DEB_STATUS_NONFINITE;
+      break;
+    }
+    const bool keep = nrm < 1.0;
+    double inv = 1.0 / nrm;
+    double f1 = coeff1 == 0.0 ? 1.0 : pow(inv, coeff1);
+    double f2 = coeff2 == 0.0 ? 1.0 : pow(prev_inv, coeff2);
+    double f3 = coeff3 == 0.0 ? 1.0 : pow(prev_prev_inv, coeff3);
+    double fmn = keep ? 1.0 : cf.factormin;
+    double factor = fmin(fmax(0.9 * f1 * f2 * f3, fmn), cf.factormax);
+    double dtn = dt * factor;
+    if (inv == 0.0 || isinf(inv)) inv = 1.0;
+    if (keep) {
+      while (iout < P.nout && tout[iout] <= tnext) {
+        const double th = (tout[iout] - tprev) / dt;
+        for (int b = 0; b < B; ++b) {
+          cx.k = P.kmodes[mode0 + b];
+          const double *y0p = state(b, 0), *y1p = state(b, 1);
+          double ys[NS];
+#pragma unroll
+          for (int s = 0; s < NS; ++s) ys[s] = y0p[s * WAVE] + th * (y1p[s * WAVE] - y0p[s * WAVE]);
+          write_outputs<NS>(P, cx, m.rd, ys, lds, P.y_out + ((mode0 + b) * P.nout + iout) * (int64_t)width);
+        }
+        ++iout;
+      }
+      prev_prev_inv = prev_inv;
+      prev_inv = inv;
+      for (int b = 0; b < B; ++b) {
+        const double *src = state(b, 1);
+        double *dst = state(b, 0);
+#pragma unroll
+        for (int s = 0; s < NS; ++s) dst[s * WAVE] = src[s * WAVE];
+      }
+    }
+    double nt0 = keep ? tnext : tprev;
+    double nt1 = nt0 + dtn;
+    if (nt1 > t1 - 1e-10) nt1 = keep ? t1 : nt0 + 0.5 * (t1 - nt0);
+    tprev = nt0;
+    tnext = nt1;
+    ++steps;
+  }
+  if (lane == 0) {
+    for (int b = 0; b < B; ++b) {
+      P.status[mode0 + b] = status;
+      if (P.nsteps) P.nsteps[mode0 + b] = steps;
+    }
+  }
+}
+
 // One workgroup (one wavefront) per (cosmology, k) item; the hardware
 // dispatcher hands the next item to whichever SIMD frees up first, and items
 // are numbered so that the largest k (most steps) start first.

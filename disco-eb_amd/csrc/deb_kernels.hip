@@ -7,6 +7,9 @@
 //                       writes the output fields at every tau_out.
 //   deb_forward_tape_kernel  the same solve, also writing every accepted state
 //                       to the checkpoint tape of the gradient path.
+//   deb_forward_batched_kernel  one wavefront per (cosmology, group of
+//                       batch_size consecutive k): the group shares one step
+//                       sequence (evolve_perturbations_batched).
 //   deb_rhs_kernel      test hook: model_synchronous for given states.
 //   deb_step_kernel     test hook: one Rodas5Transformed step.
 #include <hip/hip_runtime.h>
@@ -24,6 +27,19 @@
 namespace deb {
 
 thread_local std::string g_err;
+
+// evolve_perturbations_batched: one wavefront per (cosmology, group of B
+// consecutive k), groups of the largest k first.
+template <int NS>
+__global__ void __launch_bounds__(WAVE) deb_forward_batched_kernel(const Params<NS> P, const int B, double *ws) {
+  __shared__ double lds[WAVE * NS];
+  const int ng = P.nk / B;
+  const int item = blockIdx.x;
+  if (item >= P.n_cosmo * ng) return;
+  const int g = ng - 1 - item / P.n_cosmo;
+  const int ci = item % P.n_cosmo;
+  integrate_batch<NS>(P, ci, g * B, B, lds, ws);
+}
 
 // The forward solve that also writes every accepted state to the checkpoint
 // tape of the gradient path (deb_solve_forward_tape).
@@ -128,6 +144,23 @@ int run_forward(const deb_config_t *cfg, const Layout &L, int n_cosmo, const deb
 }
 
 template <int NS>
+int run_forward_batched(const deb_config_t *cfg, const Layout &L, int n_cosmo, const deb_cosmo_t *cosmo,
+                        const double *tables, int64_t stride, const int32_t *desc, const double *q, const double *w,
+                        int nk, const double *kmodes, int nout, const double *tau_out, int batch, double *y_out,
+                        int32_t *status, int32_t *nsteps, void *ws, hipStream_t st) {
+  Params<NS> P;
+  if (int r = fill_params<NS>(P, cfg, L, n_cosmo, cosmo, tables, stride, desc, q, w, nk, kmodes, nout, tau_out))
+    return r;
+  P.y_out = y_out;
+  P.status = status;
+  P.nsteps = nsteps;
+  const unsigned items = (unsigned)((int64_t)n_cosmo * (nk / batch));
+  hipLaunchKernelGGL(deb_forward_batched_kernel<NS>, dim3(items), dim3(WAVE), 0, st, P, batch,
+                     static_cast<double *>(ws));
+  return hip_check(hipGetLastError(), "batched forward kernel launch");
+}
+
+template <int NS>
 int run_forward_tape(const deb_config_t *cfg, const Layout &L, int n_cosmo, const deb_cosmo_t *cosmo,
                      const double *tables, int64_t stride, const int32_t *desc, const double *q, const double *w,
                      int nk, const double *kmodes, int nout, const double *tau_out, double *y_out, int32_t *status,
@@ -199,6 +232,33 @@ int deb_solve_forward(const deb_config_t *cfg, int n_cosmo, const deb_cosmo_t *c
   hipStream_t st = (hipStream_t)stream;
   DEB_DISPATCH(L.ns, (run_forward<NS>(cfg, L, n_cosmo, cosmo, tables, table_stride, desc, q, w, nk, kmodes, nout,
                                       tau_out, y_out, status, nsteps, workspace, st)))
+}
+
+int deb_batched_workspace_bytes(const deb_config_t *cfg, int n_cosmo, int nk, size_t *bytes) {
+  if (!cfg || !bytes || n_cosmo < 1 || nk < 1) return fail(DEB_E_ARG, "invalid argument");
+  Layout L;
+  if (int r = build_layout(*cfg, L)) return r;
+  *bytes = (size_t)n_cosmo * (size_t)nk * 2 * (size_t)L.ns * WAVE * sizeof(double);
+  return 0;
+}
+
+int deb_solve_forward_batched(const deb_config_t *cfg, int n_cosmo, const deb_cosmo_t *cosmo, const double *tables,
+                              int64_t table_stride, const int32_t *desc, const double *q, const double *w, int nk,
+                              const double *kmodes, int nout, const double *tau_out, int batch_size, double *y_out,
+                              int32_t *status, int32_t *nsteps, void *workspace, size_t workspace_bytes,
+                              void *stream) {
+  if (int r = check_common(cfg, n_cosmo, cosmo, tables, desc, q, w)) return r;
+  if (nk < 1 || nout < 1 || !kmodes || !tau_out || !y_out || !status) return fail(DEB_E_ARG, "invalid mode arrays");
+  if (batch_size < 1 || nk % batch_size != 0) return fail(DEB_E_ARG, "num_k must be a multiple of batch_size");
+  if ((int64_t)n_cosmo * nk > 0x7fffffff) return fail(DEB_E_ARG, "too many modes");
+  size_t need = 0;
+  if (int r = deb_batched_workspace_bytes(cfg, n_cosmo, nk, &need)) return r;
+  if (!workspace || workspace_bytes < need) return fail(DEB_E_ARG, "workspace smaller than deb_batched_workspace_bytes");
+  Layout L;
+  if (int r = build_layout(*cfg, L)) return r;
+  hipStream_t st = (hipStream_t)stream;
+  DEB_DISPATCH(L.ns, (run_forward_batched<NS>(cfg, L, n_cosmo, cosmo, tables, table_stride, desc, q, w, nk, kmodes,
+                                              nout, tau_out, batch_size, y_out, status, nsteps, workspace, st)))
 }
 
 int deb_solve_forward_tape(const deb_config_t *cfg, int n_cosmo, const deb_cosmo_t *cosmo, const double *tables,

@@ -47,8 +47,10 @@ class DeviceInputs:
         self.cfg = cfg
 
 
-def solve_forward(inp: DeviceInputs, kmodes, tau_out, *, throw: bool = True, stream=None):
-    """Run deb_solve_forward; kmodes (n_cosmo, nk), tau_out (n_cosmo, nout) tensors."""
+def solve_forward(inp: DeviceInputs, kmodes, tau_out, *, throw: bool = True, stream=None, batch_size: int = 0):
+    """Run deb_solve_forward; kmodes (n_cosmo, nk), tau_out (n_cosmo, nout) tensors.
+    With batch_size > 0, deb_solve_forward_batched: groups of batch_size
+    consecutive k share one step sequence (evolve_modes_batched)."""
     torch = _torch()
     L = _lib.lib()
     cfg = inp.cfg
@@ -64,14 +66,23 @@ def solve_forward(inp: DeviceInputs, kmodes, tau_out, *, throw: bool = True, str
     status = torch.full((n_cosmo, nk), -1, dtype=torch.int32, device=inp.device)
     nsteps = torch.zeros((n_cosmo, nk), dtype=torch.int32, device=inp.device)
     wsb = ctypes.c_size_t(0)
-    _lib.check(L.deb_workspace_bytes(ctypes.byref(cfg), n_cosmo, nk, nout, ctypes.byref(wsb)), "deb_workspace_bytes")
-    ws = torch.empty(int(wsb.value), dtype=torch.uint8, device=inp.device)
     st = stream if stream is not None else torch.cuda.current_stream(inp.device)
-    _lib.check(L.deb_solve_forward(ctypes.byref(cfg), n_cosmo, _lib.ptr(inp.cosmo), _lib.ptr(inp.tables),
-                                   inp.stride, _lib.ptr(inp.desc), _lib.ptr(inp.q), _lib.ptr(inp.w), nk,
-                                   _lib.ptr(kmodes), nout, _lib.ptr(tau_out), _lib.ptr(y), _lib.ptr(status),
-                                   _lib.ptr(nsteps), _lib.ptr(ws), wsb.value, ctypes.c_void_p(st.cuda_stream)),
-               "deb_solve_forward")
+    common = (ctypes.byref(cfg), n_cosmo, _lib.ptr(inp.cosmo), _lib.ptr(inp.tables), inp.stride, _lib.ptr(inp.desc),
+              _lib.ptr(inp.q), _lib.ptr(inp.w), nk, _lib.ptr(kmodes), nout, _lib.ptr(tau_out))
+    if batch_size > 0:
+        _lib.check(L.deb_batched_workspace_bytes(ctypes.byref(cfg), n_cosmo, nk, ctypes.byref(wsb)),
+                   "deb_batched_workspace_bytes")
+        ws = torch.empty(int(wsb.value), dtype=torch.uint8, device=inp.device)
+        _lib.check(L.deb_solve_forward_batched(*common, batch_size, _lib.ptr(y), _lib.ptr(status), _lib.ptr(nsteps),
+                                               _lib.ptr(ws), wsb.value, ctypes.c_void_p(st.cuda_stream)),
+                   "deb_solve_forward_batched")
+    else:
+        _lib.check(L.deb_workspace_bytes(ctypes.byref(cfg), n_cosmo, nk, nout, ctypes.byref(wsb)),
+                   "deb_workspace_bytes")
+        ws = torch.empty(int(wsb.value), dtype=torch.uint8, device=inp.device)
+        _lib.check(L.deb_solve_forward(*common, _lib.ptr(y), _lib.ptr(status), _lib.ptr(nsteps), _lib.ptr(ws),
+                                       wsb.value, ctypes.c_void_p(st.cuda_stream)),
+                   "deb_solve_forward")
     if throw:
         st.synchronize()
         bad = (status != STATUS_OK)
@@ -132,17 +143,25 @@ def evolve_perturbations_batched(*, param, aexp_out, kmin: float, kmax: float, n
                                  rtol: float = 1e-4, atol: float = 1e-4, pcoeff: float = 0.25, icoeff: float = 0.80,
                                  dcoeff: float = 0.0, factormax: float = 20.0, factormin: float = 0.3,
                                  max_steps: int = 2048, batch_size: int = 16, throw: bool = True):
-    """Batched-API mirror: geomspace k-grid, ``num_k % batch_size == 0``, 2-tuple
-    return ``(y, kmodes)``.  Every mode keeps its own step sequence (the
-    reference shares one step sequence per group of batch_size modes)."""
+    """evolve_perturbations_batched (perturbations.py:1000-1061): geomspace
+    k-grid split into groups of batch_size consecutive modes, each group
+    integrated with one shared step sequence (evolve_modes_batched :820-922);
+    ``num_k % batch_size == 0``; returns ``(y, kmodes)``."""
+    torch = _torch()
     if num_k % batch_size != 0:
         raise ValueError("num_k must be a multiple of batch_size")
-    y, k, p = evolve_perturbations(param=param, aexp_out=aexp_out, kmin=kmin, kmax=kmax, num_k=num_k, lmaxg=lmaxg,
-                                   lmaxgp=lmaxgp, lmaxr=lmaxr, lmaxnu=lmaxnu, nqmax=nqmax, rtol=rtol, atol=atol,
-                                   pcoeff=pcoeff, icoeff=icoeff, dcoeff=dcoeff, factormax=factormax,
-                                   factormin=factormin, max_steps=max_steps, dologk=True, throw=throw)
-    param["nout"] = p["nout"]
-    return y, k
+    if np.asarray(param["H0"]).size != 1:
+        raise ValueError("evolve_perturbations_batched takes one cosmology")
+    cfg = _config(lmaxg, lmaxgp, lmaxr, lmaxnu, nqmax, rtol, atol, pcoeff, icoeff, dcoeff, factormax, factormin,
+                  max_steps, False)
+    kmodes = np.geomspace(kmin, kmax, num_k)
+    aexp_out = np.atleast_1d(np.asarray(aexp_out, dtype=np.float64))
+    tau_out = packing.tau_of_aexp(param, aexp_out).real
+    inp = DeviceInputs(param, cfg)
+    y, _, _ = solve_forward(inp, torch.from_numpy(kmodes[None, :]), torch.from_numpy(tau_out), throw=throw,
+                            batch_size=batch_size)
+    param["nout"] = aexp_out.size
+    return y[0], torch.from_numpy(kmodes).to(y.device)
 
 
 def evolve_perturbations_multi(*, param, aexp_out, kmodes, lmaxg: int = 11, lmaxgp: int = 11, lmaxr: int = 11,

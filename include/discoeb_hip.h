@@ -132,6 +132,25 @@ int deb_solve_forward(const deb_config_t *cfg, int n_cosmo, const deb_cosmo_t *c
                       int32_t *nsteps, void *workspace, size_t workspace_bytes, void *stream);
 
 /*
+ * Replaces evolve_perturbations_batched / evolve_modes_batched
+ * (perturbations.py:820-922, 1000-1061): the nk wavenumbers of each cosmology
+ * are split into nk / batch_size groups of consecutive k; each group shares one
+ * step sequence (start time = the group's minimum start time,
+ * determine_start_times_per_batch :786-803; the step-size controller sees the
+ * RMS over all the group's filtered, weighted errors,
+ * rms_norm_filtered_batched :690-694).  One wavefront per (cosmology, group).
+ * Arguments as deb_solve_forward; nk must be a multiple of batch_size
+ * (DEB_E_ARG otherwise); status and nsteps are the group's for every mode;
+ * workspace >= deb_batched_workspace_bytes (two states per mode).
+ */
+int deb_batched_workspace_bytes(const deb_config_t *cfg, int n_cosmo, int nk, size_t *bytes);
+int deb_solve_forward_batched(const deb_config_t *cfg, int n_cosmo, const deb_cosmo_t *cosmo, const double *tables,
+                              int64_t table_stride, const int32_t *desc, const double *q, const double *w, int nk,
+                              const double *kmodes, int nout, const double *tau_out, int batch_size, double *y_out,
+                              int32_t *status, int32_t *nsteps, void *workspace, size_t workspace_bytes,
+                              void *stream);
+
+/*
  * Operator-level test hook: f = model_synchronous(tau, y) (perturbations.py:84-371)
  * for m independent (cosmology index, k, tau, y) tuples, evaluated by the
  * kernel's wave-parallel right-hand side.

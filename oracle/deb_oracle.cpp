@@ -12,6 +12,9 @@
 //   adiabatic_ics_one_mode     src/discoeb/perturbations.py:526-627
 //   determine_starting_time    src/discoeb/perturbations.py:630-681 (+ util.py:365-396)
 //   evolve_one_mode            src/discoeb/perturbations.py:728-781
+//   evolve_modes_batched       src/discoeb/perturbations.py:786-922 (one step
+//                              sequence per group: start time = group minimum,
+//                              norm = rms_norm_filtered_batched :690-694)
 //   rms_norm_filtered          src/discoeb/perturbations.py:701-705
 //   Rodas5Transformed.step     src/discoeb/ode_integrators_stiff.py:622-834
 //   get_power                  src/discoeb/perturbations.py:1101-1123
@@ -729,6 +732,91 @@ int evolve_mode(const Model &m, const SolverCfg &s, const double *tau_out, int n
   return status;
 }
 
+// evolve_modes_batched for one group of modes (perturbations.py:820-922):
+// the group's start time is the minimum of its modes' start times
+// (determine_start_times_per_batch :786-803), every mode starts from its ICs
+// there, and one step sequence serves the group: the controller sees the RMS
+// over all modes' filtered, weighted errors (rms_norm_filtered_batched
+// :690-694).  out[b][nout][width]; every mode gets the group's status.
+int evolve_batch(const std::vector<Model> &ms, const SolverCfg &s, const double *tau_out, int nout, double *out,
+                 int *nsteps, int *naccept) {
+  const int B = (int)ms.size();
+  const int n = ms[0].nvar();
+  Work<double> w(n);
+  double tmin = tau_out[0], tmax = tau_out[0];
+  for (int i = 1; i < nout; ++i) {
+    tmin = std::min(tmin, tau_out[i]);
+    tmax = std::max(tmax, tau_out[i]);
+  }
+  double t0 = start_time<double>(ms[0], tmin);
+  for (int b = 1; b < B; ++b) t0 = std::min(t0, start_time<double>(ms[b], tmin));
+  const double t1 = tmax;
+  std::vector<std::vector<double>> y(B, std::vector<double>(n)), y1 = y;
+  std::vector<double> err(n), ys(n);
+  for (int b = 0; b < B; ++b) ics<double>(ms[b], t0, y[b].data());
+  const double dt0 = std::min(t0 / 4.0, 0.5 * (t1 - t0));
+  double tprev = t0, tnext = std::min(t0 + dt0, t1);
+  double prev_inv = 1.0, prev_prev_inv = 1.0;
+  const double eo = 5.0;
+  const double coeff1 = (s.icoeff + s.pcoeff + s.dcoeff) / eo;
+  const double coeff2 = -(s.pcoeff + 2.0 * s.dcoeff) / eo;
+  const double coeff3 = s.dcoeff / eo;
+  const int fidx[6] = {0, 2, 3, 5, 6, 7};
+  int iout = 0, steps = 0, acc = 0;
+  const int width = s.return_full ? n : 20;
+  int status = DEB_STATUS_OK;
+  while (tprev < t1) {
+    if (steps >= s.max_steps) { status = DEB_STATUS_MAX_STEPS; break; }
+    double ss = 0.0;
+    for (int b = 0; b < B; ++b) {
+      rodas5_step<double, double>(ms[b], tprev, tnext, y[b].data(), y1[b].data(), err.data(), w);
+      const double fw[6] = {1.0, ms[b].k * ms[b].k, 1.0, 1.0, 1.0 / (ms[b].k * ms[b].k), 1.0};
+      for (int j = 0; j < 6; ++j) {
+        int i = fidx[j];
+        double sc = err[i] / (s.atol + std::max(std::fabs(y[b][i]), std::fabs(y1[b][i])) * s.rtol);
+        double x = sc * fw[j];
+        ss += x * x;
+      }
+    }
+    double nrm = std::sqrt(ss / (6.0 * B));
+    if (!std::isfinite(nrm)) { status = DEB_STATUS_NONFINITE; break; }
+    bool keep = nrm < 1.0;
+    double inv = 1.0 / nrm;
+    double f1 = coeff1 == 0.0 ? 1.0 : std::pow(inv, coeff1);
+    double f2 = coeff2 == 0.0 ? 1.0 : std::pow(prev_inv, coeff2);
+    double f3 = coeff3 == 0.0 ? 1.0 : std::pow(prev_prev_inv, coeff3);
+    double fmin = keep ? 1.0 : s.factormin;
+    double factor = std::min(std::max(0.9 * f1 * f2 * f3, fmin), s.factormax);
+    double dt = (tnext - tprev) * factor;
+    if (inv == 0.0 || std::isinf(inv)) inv = 1.0;
+    if (keep) {
+      while (iout < nout && tau_out[iout] <= tnext) {
+        double th = (tau_out[iout] - tprev) / (tnext - tprev);
+        for (int b = 0; b < B; ++b) {
+          for (int i = 0; i < n; ++i) ys[i] = y[b][i] + th * (y1[b][i] - y[b][i]);
+          double *o = out + ((size_t)b * nout + iout) * width;
+          if (s.return_full) std::copy(ys.begin(), ys.end(), o);
+          else outputs<double, double>(ms[b], ys.data(), o);
+        }
+        ++iout;
+      }
+      prev_prev_inv = prev_inv;
+      prev_inv = inv;
+      for (int b = 0; b < B; ++b) y[b].swap(y1[b]);
+      ++acc;
+    }
+    double nt0 = keep ? tnext : tprev;
+    double nt1 = nt0 + dt;
+    if (nt1 > t1 - 1e-10) nt1 = keep ? t1 : nt0 + 0.5 * (t1 - nt0);
+    tprev = nt0;
+    tnext = nt1;
+    ++steps;
+  }
+  *nsteps = steps;
+  *naccept = acc;
+  return status;
+}
+
 // directional derivative of the 20 output fields along one tangent of the
 // inputs, replaying the accepted step end times `times` (t0 first) with each
 // step held fixed as a fraction of [t0, t1] (see the file header)
@@ -947,6 +1035,43 @@ int oracle_evolve(const deb_config_t *cfg, const deb_cosmo_t *c, const double *t
       nsteps[i] = st;
       naccept[i] = na;
       if (tau_start) tau_start[i] = ts;
+    }
+  };
+  if (nthreads <= 1) {
+    worker();
+  } else {
+    std::vector<std::thread> th;
+    for (int t = 0; t < nthreads; ++t) th.emplace_back(worker);
+    for (auto &t : th) t.join();
+  }
+  return 0;
+}
+
+// evolve_modes_batched: groups of `batch` consecutive k (nk % batch == 0)
+int oracle_evolve_batched(const deb_config_t *cfg, const deb_cosmo_t *c, const double *tables, const int32_t *desc,
+                          const double *q, const double *w, int nk, const double *kmodes, int nout,
+                          const double *tau_out, int batch, double *y_out, int32_t *status, int32_t *nsteps,
+                          int32_t *naccept, int nthreads) {
+  if (batch < 1 || nk % batch != 0) return -1;
+  SolverCfg s{cfg->rtol, cfg->atol, cfg->pcoeff, cfg->icoeff, cfg->dcoeff, cfg->factormax, cfg->factormin,
+              cfg->max_steps, cfg->return_full};
+  const int n = oracle_nvar(cfg);
+  const size_t width = cfg->return_full ? (size_t)n : 20;
+  const int ng = nk / batch;
+  std::atomic<int> next(0);
+  auto worker = [&]() {
+    for (;;) {
+      int g = next.fetch_add(1);
+      if (g >= ng) break;
+      std::vector<Model> ms;
+      for (int b = 0; b < batch; ++b) ms.push_back(make_model(c, tables, desc, cfg, q, w, kmodes[g * batch + b]));
+      int st, na;
+      const int r = evolve_batch(ms, s, tau_out, nout, y_out + (size_t)g * batch * nout * width, &st, &na);
+      for (int b = 0; b < batch; ++b) {
+        status[g * batch + b] = r;
+        nsteps[g * batch + b] = st;
+        naccept[g * batch + b] = na;
+      }
     }
   };
   if (nthreads <= 1) {

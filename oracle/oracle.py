@@ -118,6 +118,23 @@ class Problem:
                             _p(st, _ip), _p(ns, _ip), _p(na, _ip), _p(ts), ctypes.c_int(nthreads))
         return dict(y=y, status=st, nsteps=ns, naccept=na, tau_start=ts)
 
+    def evolve_batched(self, kmodes, tau_out, batch: int, nthreads: int = 1):
+        """evolve_modes_batched: groups of `batch` consecutive k share one step sequence."""
+        kmodes = np.ascontiguousarray(kmodes, dtype=np.float64)
+        tau_out = np.ascontiguousarray(np.atleast_1d(tau_out), dtype=np.float64)
+        nk, nout = kmodes.size, tau_out.size
+        width = self.n if self.cfg.return_full else 20
+        y = np.zeros((nk, nout, width))
+        st = np.zeros(nk, dtype=np.int32)
+        ns = np.zeros(nk, dtype=np.int32)
+        na = np.zeros(nk, dtype=np.int32)
+        r = lib().oracle_evolve_batched(*self._common(), ctypes.c_int(nk), _p(kmodes), ctypes.c_int(nout),
+                                        _p(tau_out), ctypes.c_int(batch), _p(y), _p(st, _ip), _p(ns, _ip),
+                                        _p(na, _ip), ctypes.c_int(nthreads))
+        if r != 0:
+            raise ValueError("num_k must be a multiple of batch")
+        return dict(y=y, status=st, nsteps=ns, naccept=na)
+
     # ---- tangents along one input direction (dcos[24], dtab[stride]) ----
     def _tan(self, dcos, dtab):
         dcos = np.ascontiguousarray(dcos, dtype=np.float64)

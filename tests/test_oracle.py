@@ -243,3 +243,28 @@ def test_rodas5_tableau_row_sums():
         assert abs(got - want) < 1e-12, (got, want)
     ci = ai
     np.testing.assert_allclose(ci[1:5], c[1:5], rtol=1e-12)
+
+
+def test_batched_evolve_semantics(test_param):
+    """evolve_modes_batched restated: a group of one mode is the per-mode solve
+    (bit for bit); in larger groups every mode carries the group's step count
+    and the solution agrees with the per-mode one at the tolerance level."""
+    cfg = packing.make_config(rtol=1e-4, atol=1e-4, max_steps=20000, lmaxg=11, lmaxgp=11, lmaxr=11, lmaxnu=8,
+                              nqmax=3)
+    prob = oracle.Problem(test_param, cfg)
+    k = np.geomspace(1e-3, 0.5, 8)
+    tau = packing.tau_of_aexp(test_param, [0.1, 1.0])[0]
+    single = prob.evolve(k, tau)
+    one = prob.evolve_batched(k, tau, 1)
+    np.testing.assert_array_equal(one["y"], single["y"])
+    np.testing.assert_array_equal(one["nsteps"], single["nsteps"])
+    four = prob.evolve_batched(k, tau, 4)
+    assert np.all(four["status"] == 0)
+    for g in range(2):
+        grp = slice(4 * g, 4 * g + 4)
+        assert len(set(four["nsteps"][grp].tolist())) == 1
+    # same solution to the tolerance level
+    rel = np.abs(four["y"][:, -1, 4] - single["y"][:, -1, 4]) / np.abs(single["y"][:, -1, 4])
+    assert rel.max() < 3e-3, rel.max()
+    with pytest.raises(ValueError):
+        prob.evolve_batched(k, tau, 3)
