@@ -440,9 +440,9 @@ __device__ __forceinline__ int spl_interval(const Spl &s, double v, int &hint) {
   return i;
 }
 
+// cubic spline piece i at xv (knots x, y, S of s)
 template <typename T>
-__device__ __forceinline__ T spl_eval(const Spl &s, T xv, int &hint) {
-  int i = spl_interval(s, vval(xv), hint);
+__device__ __forceinline__ T spl_piece(const Spl &s, int i, T xv) {
   double x0 = s.x[i], x1 = s.x[i + 1];
   double h = x1 - x0;
   T t = (xv - x0) / h;
@@ -450,6 +450,35 @@ __device__ __forceinline__ T spl_eval(const Spl &s, T xv, int &hint) {
   T B = t;
   double y0 = s.y[i], y1 = s.y[i + 1], s0 = s.S[i], s1 = s.S[i + 1];
   return A * y0 + B * y1 + ((A * A * A - A) * s0 + (B * B * B - B) * s1) * (h * h / 6.0);
+}
+
+// three splines at one abscissa (the same intervals as spl_interval): the
+// hinted intervals' end knots of all three are loaded before any is tested,
+// so their latencies overlap
+template <typename T>
+__device__ __forceinline__ void spl_eval3(const Spl (&sp)[3], T xv, int *const (&hint)[3], T (&r)[3]) {
+  const double v = vval(xv);
+  int i[3];
+  double lo[3], hi[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    i[j] = *hint[j];
+    const int c = i[j] < 0 ? 0 : (i[j] > sp[j].n - 2 ? sp[j].n - 2 : i[j]);
+    lo[j] = sp[j].x[c];
+    hi[j] = sp[j].x[c + 1];
+  }
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const bool ok = i[j] >= 0 && i[j] <= sp[j].n - 2 && lo[j] < v && v <= hi[j];
+    if (!ok) i[j] = spl_interval(sp[j], v, *hint[j]);
+  }
+#pragma unroll
+  for (int j = 0; j < 3; ++j) r[j] = spl_piece(sp[j], i[j], xv);
+}
+
+template <typename T>
+__device__ __forceinline__ T spl_eval(const Spl &s, T xv, int &hint) {
+  return spl_piece(s, spl_interval(s, vval(xv), hint), xv);
 }
 
 }  // namespace deb

@@ -113,9 +113,22 @@ __device__ __forceinline__ Bg<T> background(Ctx &cx, T a) {
   Bg<T> b;
   b.a = a;
   T loga = dlog(a);
-  b.cs2 = spl_at<T>(cx, DEB_SPL_CS2A_LOGA, loga) / a;
-  T xe = spl_at<T>(cx, DEB_SPL_XE_LOGA, loga);
-  T rhonu = dexp(spl_at<T>(cx, DEB_SPL_LOGRHONU_LOGA, loga));
+  T cs2a, xe, lrhonu;
+  if constexpr (is_dn<T>::value) {
+    cs2a = spl_at<T>(cx, DEB_SPL_CS2A_LOGA, loga);
+    xe = spl_at<T>(cx, DEB_SPL_XE_LOGA, loga);
+    lrhonu = spl_at<T>(cx, DEB_SPL_LOGRHONU_LOGA, loga);
+  } else {
+    const Spl sp3[3] = {cx.sp[DEB_SPL_CS2A_LOGA], cx.sp[DEB_SPL_XE_LOGA], cx.sp[DEB_SPL_LOGRHONU_LOGA]};
+    int *const h3[3] = {&cx.hint[DEB_SPL_CS2A_LOGA], &cx.hint[DEB_SPL_XE_LOGA], &cx.hint[DEB_SPL_LOGRHONU_LOGA]};
+    T r3[3];
+    spl_eval3<T>(sp3, loga, h3, r3);
+    cs2a = r3[0];
+    xe = r3[1];
+    lrhonu = r3[2];
+  }
+  b.cs2 = cs2a / a;
+  T rhonu = dexp(lrhonu);
   b.pb43 = (4.0 / 3.0) * (grhog / (grhom * Omegab)) / a;
   b.w = w0 + wa * (1.0 - a);
   b.rhoQ = rho_de<T>(a, loga, w0, wa);
