@@ -221,6 +221,12 @@ def main():
     st = last["st"].cpu().numpy()
     ns = last["ns"].cpu().numpy()
     n_fail = int((st != 0).sum())
+    if n_fail:
+        # a mode that stopped early did less work than the workload asks for:
+        # no throughput is reported for such a run
+        vals, cnt = np.unique(st[st != 0], return_counts=True)
+        sys.exit(f"bench: {n_fail} of {st.size} modes ended with a nonzero status "
+                 f"({dict(zip(vals.tolist(), cnt.tolist()))}); no result reported")
     total_steps = float(ns.sum())
     kern_ms = {k: float(np.mean(v)) for k, v in kern.items()}
     value = nc / (ms_per_step * 1e-3)
@@ -279,10 +285,12 @@ def plain_baseline(cfg, thetas, nk, dev, adjoint_value):
         prepared = [make_inputs(t, cfg, dev, nk, False) for t in ths]
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for (_, _, inp, k_t, tau_t, _) in prepared:
-            solve_forward(inp, k_t, tau_t, throw=False)
+        outs = [solve_forward(inp, k_t, tau_t, throw=False) for (_, _, inp, k_t, tau_t, _) in prepared]
         torch.cuda.synchronize()
         per.append(time.perf_counter() - t0)
+        n_fail = sum(int((st != 0).sum().item()) for (_, st, _) in outs)
+        if n_fail:
+            sys.exit(f"bench: plain baseline: {n_fail} modes ended with a nonzero status; no result reported")
     sec = float(np.mean(per))
     return {"value": 1.0 / sec, "unit": "(P,dP/dtheta)/s", "launches_per_cosmology": 13,
             "sample": f"{len(thetas)} cosmologies, each 13 launches of 1 cosmology x {nk} k",
