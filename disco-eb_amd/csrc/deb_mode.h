@@ -175,6 +175,14 @@ __device__ __forceinline__ void row_visit(uint32_t d, const Bg<T> &b, const Ctx 
   const T a2 = a * a;
   const P Omegac = Omegam - Omegab;
   const T inv_aH = 1.0 / b.aH;
+  // multipole couplings l/(2l+1) and (l+1)/(2l+1), and the massive-nu velocity
+  // of the row's momentum bin: evaluated once here rather than in each of the
+  // (divergent) row cases that use them
+  const double il = 1.0 / (2.0 * l + 1.0);
+  const double lsub = l * il, lsup = (l + 1.0) * il;
+  const bool qrow = t >= R_Q0 && t <= R_QLMAX;
+  T vq = tconst<T>(1.0);
+  if (qrow) vq = nu_v<T>(cx, a, qi);
   switch (t) {
     case R_ETA:
       sink(std::integral_constant<int, C_UETA>{}, 1.0);
@@ -225,9 +233,9 @@ __device__ __forceinline__ void row_visit(uint32_t d, const Bg<T> &b, const Ctx 
       sink(std::integral_constant<int, C_CPI>{}, 1.0);
       break;
     case R_FL:
-      sink(std::integral_constant<int, C_SUB>{}, k * l / (2.0 * l + 1.0));
+      sink(std::integral_constant<int, C_SUB>{}, k * lsub);
       sink(std::integral_constant<int, C_DIAG>{}, -b.opac);
-      sink(std::integral_constant<int, C_SUP>{}, -k * (l + 1.0) / (2.0 * l + 1.0));
+      sink(std::integral_constant<int, C_SUP>{}, -k * lsup);
       break;
     case R_FLMAX:
       sink(std::integral_constant<int, C_SUB>{}, k);
@@ -240,9 +248,9 @@ __device__ __forceinline__ void row_visit(uint32_t d, const Bg<T> &b, const Ctx 
       sink(std::integral_constant<int, C_CPI>{}, 1.0);
       break;
     case R_GL:
-      sink(std::integral_constant<int, C_SUB>{}, k * l / (2.0 * l + 1.0));
+      sink(std::integral_constant<int, C_SUB>{}, k * lsub);
       sink(std::integral_constant<int, C_DIAG>{}, -b.opac);
-      sink(std::integral_constant<int, C_SUP>{}, -k * (l + 1.0) / (2.0 * l + 1.0));
+      sink(std::integral_constant<int, C_SUP>{}, -k * lsup);
       break;
     case R_G2:
       sink(std::integral_constant<int, C_SUB>{}, k * 2.0 / 5.0);
@@ -273,15 +281,14 @@ __device__ __forceinline__ void row_visit(uint32_t d, const Bg<T> &b, const Ctx 
       sink(std::integral_constant<int, C_UETA>{}, 8.0 / 5.0);
       break;
     case R_FRL:
-      sink(std::integral_constant<int, C_SUB>{}, k * l / (2.0 * l + 1.0));
-      sink(std::integral_constant<int, C_SUP>{}, -k * (l + 1.0) / (2.0 * l + 1.0));
+      sink(std::integral_constant<int, C_SUB>{}, k * lsub);
+      sink(std::integral_constant<int, C_SUP>{}, -k * lsup);
       break;
     case R_FRLMAX:
       sink(std::integral_constant<int, C_SUB>{}, k);
       sink(std::integral_constant<int, C_DIAG>{}, -(l + 1.0) * invtau);
       break;
     case R_Q0: {
-      T vq = nu_v<T>(cx, a, qi);
       T kv = k * vq;
       sink(std::integral_constant<int, C_SUP>{}, -kv);
       sink(std::integral_constant<int, C_UH>{}, cx.dlnf[qi] / 6.0);
@@ -290,7 +297,6 @@ __device__ __forceinline__ void row_visit(uint32_t d, const Bg<T> &b, const Ctx 
       break;
     }
     case R_Q1: {
-      T vq = nu_v<T>(cx, a, qi);
       T kv = k * vq;
       sink(std::integral_constant<int, C_SUB>{}, kv / 3.0);
       sink(std::integral_constant<int, C_SUP>{}, -2.0 * kv / 3.0);
@@ -298,7 +304,6 @@ __device__ __forceinline__ void row_visit(uint32_t d, const Bg<T> &b, const Ctx 
       break;
     }
     case R_Q2: {
-      T vq = nu_v<T>(cx, a, qi);
       T kv = k * vq;
       sink(std::integral_constant<int, C_SUB>{}, 2.0 * kv / 5.0);
       sink(std::integral_constant<int, C_SUP>{}, -3.0 * kv / 5.0);
@@ -307,14 +312,12 @@ __device__ __forceinline__ void row_visit(uint32_t d, const Bg<T> &b, const Ctx 
       break;
     }
     case R_QL: {
-      T vq = nu_v<T>(cx, a, qi);
       T kv = k * vq;
-      sink(std::integral_constant<int, C_SUB>{}, kv * (l / (2.0 * l + 1.0)));
-      sink(std::integral_constant<int, C_SUP>{}, -kv * ((l + 1.0) / (2.0 * l + 1.0)));
+      sink(std::integral_constant<int, C_SUB>{}, kv * lsub);
+      sink(std::integral_constant<int, C_SUP>{}, -kv * lsup);
       break;
     }
     case R_QLMAX: {
-      T vq = nu_v<T>(cx, a, qi);
       sink(std::integral_constant<int, C_SUB>{}, k * vq);
       sink(std::integral_constant<int, C_DIAG>{}, -(l + 1.0) * invtau);
       break;
