@@ -1,15 +1,17 @@
 """Benchmark of the perturbation hot path on one MI355X.
 
-Workload: 64 cosmologies drawn with seed 20261202 from the (omega_c, omega_b,
-h, n_s, ln10As, m_nu, w0, wa) box x 256 k log-spaced in [1e-5, 10] /Mpc,
-photon / polarization / massless-nu lmax 32, one massive nu (lmax 17, 5
-momentum bins), Rodas5 at rtol 1e-5, atol 1e-8, P(k) at a = 1.
+Default workload (batched-64cosmo-256k, the headline metric): 64 cosmologies
+drawn with seed 20261202 from the (omega_c, omega_b, h, n_s, ln10As, m_nu, w0,
+wa) box x 256 k log-spaced in [1e-5, 10] /Mpc, photon / polarization /
+massless-nu lmax 32, one massive nu (lmax 17, 5 momentum bins), Rodas5 at
+rtol 1e-5, atol 1e-8, P(k) at a = 1.
 
 A "step" returns P(k) and dP(k)/dtheta (all 8 parameters) for all 64
 cosmologies x 256 k:
-  --arm adjoint (default) one taped forward launch over all 16384 modes, then
-                one reverse launch (batched adjoint); six directions come from
-                the reverse launch, n_s and ln10As analytically
+  --arm adjoint (default) one taped forward launch over all 16384 modes
+                (checkpoints every 64 accepted steps), then one reverse launch
+                (batched adjoint); six directions come from the reverse
+                launch, n_s and ln10As analytically
   --arm fd      one launch over the 64 base cosmologies and their +-1e-4
                 relative perturbations of omega_c, omega_b, h, m_nu, w0, wa
                 (13 solves per cosmology)
@@ -17,6 +19,16 @@ cosmologies x 256 k:
 The "plain" baseline of the metric (one cosmology per launch, forward + central
 differences, 13 launches per cosmology) is timed on a sample of cosmologies
 beside the adjoint arm and reported under "plain_baseline".
+
+Other workloads (one JSON line each, measured on request):
+  --workload batched-adjoint-256cosmo-512k  256 seeded cosmologies x 512 k in
+        [1e-5, 20], lmax 50: loss sum_k w(k) ln P(k) with seeded w(k) ~ U[0, 1]
+        and its gradient (checkpoints every 64 steps, 64 GiB device budget)
+  --workload plain-lcdm-64k          one flat LCDM cosmology (Lambda, no
+        massive nu), lmax 16, 64 k in [1e-4, 1]: P(k)
+  --workload plain-mnu-256k-16out    the base cosmology with one massive nu,
+        w0 -0.9, wa 0.1, lmax 32, 256 k, all fields at 16 a in [0.01, 1]
+
 Inputs (background tables and their theta-tangents) are built and copied to
 HBM before timing.  Prints one JSON line.
 """
@@ -36,53 +48,16 @@ sys.path.insert(0, os.path.join(ROOT, "disco-eb_amd"))
 METRIC = "(P(k), dP/dtheta) per sec, 64 cosmo x 256 k, rtol 1e-5: batched adjoint vs plain"
 FD_PARAMS = (0, 1, 2, 5, 6, 7)  # omega_c, omega_b, h, m_nu, w0, wa
 FD_REL = 1e-4
-PEAK_FP64 = 78.6  # TFLOP/s, MI355X dense FP64 vector
-
-
-def n_state(cfg) -> int:
-    return 12 + cfg.lmaxg + cfg.lmaxgp + cfg.lmaxr + cfg.nqmax * (cfg.lmaxnu + 1)
-
-
-def pcr_levels(cfg) -> int:
-    maxlen = max(cfg.lmaxg + 2, cfg.lmaxgp + 1, cfg.lmaxr + 1, cfg.lmaxnu + 1 if cfg.nqmax else 0)
-    return int(np.ceil(np.log2(maxlen)))
-
-
-def flops_per_step(cfg) -> float:
-    """FP64 flops of one attempted Rodas5 step of one mode in the structured
-    algorithm as executed (n state entries, L PCR levels): 8 right-hand sides
-    and one dual (value + d/da) right-hand side for J, the PCR factorisation
-    and Woodbury set-up, 8 structured solves and the stage combinations."""
-    n, L = n_state(cfg), pcr_levels(cfg)
-    f_bg = 150.0                       # background at one scale factor
-    f_rhs = 14.0 * n + f_bg            # per-row stencil + metric terms + 4 functionals
-    f_solve = (5.0 * L + 15) * n       # a-row, PCR sweep, Woodbury update, a*h' row
-    f_factor = 2 * f_rhs + 3 * n + 14.0 * L * n + 3 * f_solve + 24 * n
-    f_comb = 2 * 2 * sum(range(8)) * n
-    return 8 * f_rhs + f_factor + 8 * f_solve + f_comb + 20 * 6
-
-
-def flops_per_reverse_step(cfg) -> float:
-    """FP64 flops of the reverse of one accepted step: the recomputed forward
-    step, 8 transposed solves, per stage one transposed right-hand side and two
-    right-hand-side contractions in 8-component dual arithmetic (stage point and
-    J(t0, y0) k_i), and once per step a dual transposed right-hand side and a
-    16-component (nested dual) contraction for the a-column of J."""
-    n = n_state(cfg)
-    f_bg = 150.0
-    f_rhs = 14.0 * n + f_bg
-    f_solve = (5.0 * pcr_levels(cfg) + 15) * n
-    f_d8 = 3.0 * 8 * f_rhs             # dn<7>: ~3 flops per component per rhs flop
-    f_d16 = 3.0 * 16 * f_rhs
-    per_stage = f_solve + f_rhs + 2 * f_d8 + 12 * n
-    return flops_per_step(cfg) + 8 * per_stage + 2 * f_rhs + f_d16
+PEAK_FP64 = 78.6  # TFLOP/s, MI355X dense FP64 (vector and matrix)
+LMAX32 = dict(lmaxg=32, lmaxgp=32, lmaxr=32, lmaxnu=17, nqmax=5)
+WORKLOADS = ("batched-64cosmo-256k", "batched-adjoint-256cosmo-512k", "plain-lcdm-64k", "plain-mnu-256k-16out")
 
 
 def progress(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def make_inputs(theta, cfg, dev, nk, with_tangents):
+def make_inputs(theta, cfg, dev, kmodes, aexp=(1.0,), with_tangents=False, massive_nu=True):
     import torch
     from discoeb_amd import packing, tangents
     from discoeb_amd.background import evolve_background
@@ -91,16 +66,15 @@ def make_inputs(theta, cfg, dev, nk, with_tangents):
     from discoeb_amd.perturbations import DeviceInputs
     with np.errstate(all="ignore"):
         if with_tangents:
-            param, dcos, dtab, dtau = tangents.input_tangents(theta, [1.0])
+            param, dcos, dtab, dtau = tangents.input_tangents(theta, list(aexp), massive_nu=massive_nu)
         else:
-            param = evolve_background(param=param_from_theta(theta))
+            param = evolve_background(param=param_from_theta(theta, massive_nu))
     B = theta.shape[0]
-    kmodes = np.geomspace(1e-5, 10.0, nk)
     inp = DeviceInputs(param, cfg, dev)
-    k_t = torch.from_numpy(np.ascontiguousarray(np.broadcast_to(kmodes, (B, nk)))).to(dev)
-    tau_t = torch.from_numpy(np.ascontiguousarray(packing.tau_of_aexp(param, [1.0]).real)).to(dev)
+    k_t = torch.from_numpy(np.ascontiguousarray(np.broadcast_to(kmodes, (B, kmodes.size)))).to(dev)
+    tau_t = torch.from_numpy(np.ascontiguousarray(packing.tau_of_aexp(param, list(aexp)).real)).to(dev)
     tan = Tangents(dcos, dtab, dtau, dev) if with_tangents else None
-    return param, kmodes, inp, k_t, tau_t, tan
+    return param, inp, k_t, tau_t, tan
 
 
 def prefactor(theta_t, kk):
@@ -110,14 +84,34 @@ def prefactor(theta_t, kk):
     return 2 * np.pi**2 * A_s * (kk / 0.05) ** (n_s - 1) * kk ** (-3)
 
 
-def timed(fn, stream):
-    import torch
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    r = fn()
-    e1.record(stream)
-    return r, (e0, e1)
+class Timer:
+    """HIP events on the launch stream around one launch."""
+
+    def __init__(self, stream):
+        import torch
+        self.stream = stream
+        self.e0 = torch.cuda.Event(enable_timing=True)
+        self.e1 = torch.cuda.Event(enable_timing=True)
+
+    def __enter__(self):
+        self.e0.record(self.stream)
+        return self
+
+    def __exit__(self, *a):
+        self.e1.record(self.stream)
+
+    def ms(self):
+        return self.e0.elapsed_time(self.e1)
+
+
+def traffic_record(kernel, workload):
+    """HBM bytes per launch of `kernel` from the committed FETCH_SIZE /
+    WRITE_SIZE summary (profiles/traffic_<kernel>_<workload>.json), or None."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
 
 
 def main():
@@ -125,29 +119,65 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=WORKLOADS, default=WORKLOADS[0])
     ap.add_argument("--arm", choices=["adjoint", "fd", "forward"], default="adjoint")
-    ap.add_argument("--n-cosmo", type=int, default=64)
-    ap.add_argument("--num-k", type=int, default=256)
+    ap.add_argument("--n-cosmo", type=int, default=None)
+    ap.add_argument("--num-k", type=int, default=None)
+    ap.add_argument("--ckpt-every", type=int, default=64)
     ap.add_argument("--plain-sample", type=int, default=2, help="cosmologies timed one per launch (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-modes", type=int, default=6)
+    ap.add_argument("--cpu-sample-modes", type=int, default=16)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 2 GiB device-to-device (the reference kernel for HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.workload == "batched-64cosmo-256k":
+        run_batched(args)
+    elif args.workload == "batched-adjoint-256cosmo-512k":
+        run_loss_adjoint(args)
+    else:
+        run_plain(args)
 
+
+def copy_calibration():
+    """A plain 2 GiB device-to-device copy (2^28 doubles read, 2^28 written):
+    what FETCH_SIZE and WRITE_SIZE read for it scales their readings of the
+    kernels to bytes."""
     import torch
-    from discoeb_amd import gradient, packing
-    from discoeb_amd.cosmology import sample_thetas, SEED
+    x = torch.ones(1 << 28, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_each_way": (1 << 28) * 8}))
+
+
+def _accepted(tape, n_modes):
+    import torch
+    off = 256 + 8 * n_modes
+    cnt = tape.buf[off:off + 4 * n_modes].view(torch.int32)
+    return int(cnt.sum().item()) - n_modes
+
+
+def run_batched(args):
+    import torch
+    from discoeb_amd import flops, gradient, packing
+    from discoeb_amd.cosmology import SEED, sample_thetas
     from discoeb_amd.perturbations import solve_forward
 
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream(dev)
-    kw = dict(lmaxg=32, lmaxgp=32, lmaxr=32, lmaxnu=17, nqmax=5, rtol=1e-5, atol=1e-8, max_steps=32768)
-    cfg = packing.make_config(**kw)
-    nc, nk = args.n_cosmo, args.num_k
+    cfg = packing.make_config(rtol=1e-5, atol=1e-8, max_steps=32768, **LMAX32)
+    nc = args.n_cosmo or 64
+    nk = args.num_k or 256
     th0 = sample_thetas(nc, SEED)
+    kmodes = np.geomspace(1e-5, 10.0, nk)
     t_setup = time.time()
     if args.arm == "fd":
         ths = [th0]
@@ -159,34 +189,36 @@ def main():
         theta = np.concatenate(ths)
     else:
         theta = th0
-    param, kmodes, inp, k_t, tau_t, tan = make_inputs(theta, cfg, dev, nk, args.arm == "adjoint")
-    B = theta.shape[0]
+    _, inp, k_t, tau_t, tan = make_inputs(theta, cfg, dev, kmodes, with_tangents=args.arm == "adjoint")
     th_t = torch.from_numpy(theta).to(dev)
     kk = torch.from_numpy(kmodes).to(dev)[None, :]
     pref = prefactor(th_t, kk)
     dth = torch.from_numpy(np.stack([th0[:, j] * FD_REL for j in FD_PARAMS], axis=1)).to(dev)
     setup_s = time.time() - t_setup
-    progress(f"setup done ({setup_s:.1f} s): {B} solves x {nk} k")
+    progress(f"setup done ({setup_s:.1f} s): {theta.shape[0]} solves x {nk} k")
     last = {}
-    n_states = [nc * nk * 2048]
+    n_states = [None]
 
     def step():
         if args.arm == "adjoint":
-            (y, st, ns, tape), ev_f = timed(lambda: gradient.solve_forward_tape(inp, k_t, tau_t, n_states=n_states[0]),
-                                            stream)
+            with Timer(stream) as tf:
+                y, st, ns, tape = gradient.solve_forward_tape(inp, k_t, tau_t, n_states=n_states[0],
+                                                              ckpt_every=args.ckpt_every)
             dm = y[:, :, 0, 4]
             P = pref * dm**2
             ybar = torch.zeros_like(y)
             ybar[:, :, 0, 4] = 2.0 * pref * dm
-            g, ev_a = timed(lambda: gradient.solve_adjoint(inp, tan, k_t, tau_t, ybar, st, tape), stream)
+            with Timer(stream) as ta:
+                g = gradient.solve_adjoint(inp, tan, k_t, tau_t, ybar, st, tape)
             G = torch.empty((nc, nk, 8), dtype=torch.float64, device=dev)
             for j, d in enumerate(FD_PARAMS):
                 G[:, :, d] = g[:, :, j]
             G[:, :, 3] = torch.log(kk / 0.05) * P
             G[:, :, 4] = P
-            last.update(P=P, G=G, st=st, ns=ns, ev={"forward_tape": ev_f, "adjoint": ev_a}, tape=tape.nbytes)
+            last.update(P=P, G=G, st=st, ns=ns, ev={"forward_tape": tf, "adjoint": ta}, tape=tape)
         else:
-            (y, st, ns), ev = timed(lambda: solve_forward(inp, k_t, tau_t, throw=False), stream)
+            with Timer(stream) as tf:
+                y, st, ns = solve_forward(inp, k_t, tau_t, throw=False)
             dm = y[:, :, 0, 4]
             P = pref * dm**2
             if args.arm == "fd":
@@ -197,7 +229,7 @@ def main():
                 last.update(P=P0, G=G)
             else:
                 last.update(P=P)
-            last.update(st=st, ns=ns, ev={"forward": ev})
+            last.update(st=st, ns=ns, ev={"forward": tf})
 
     for i in range(args.warmup):
         tw = time.perf_counter()
@@ -205,7 +237,7 @@ def main():
         torch.cuda.synchronize()
         progress(f"warmup step {i + 1}/{args.warmup} done ({time.perf_counter() - tw:.1f} s)")
         if args.arm == "adjoint":
-            n_states[0] = int(last["ns"].sum().item()) + nc * nk  # exact bound for the timed steps
+            n_states[0] = int(last["ns"].sum().item()) + nc * nk  # a sufficient bound for the timed steps
     kern = {}
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -213,8 +245,8 @@ def main():
         step()
         torch.cuda.synchronize()
         progress(f"timed step {i + 1}/{args.steps} done")
-        for name, (e0, e1) in last["ev"].items():
-            kern.setdefault(name, []).append(e0.elapsed_time(e1))
+        for name, tm in last["ev"].items():
+            kern.setdefault(name, []).append(tm.ms())
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     ms_per_step = 1e3 * elapsed / args.steps
@@ -227,49 +259,196 @@ def main():
         vals, cnt = np.unique(st[st != 0], return_counts=True)
         sys.exit(f"bench: {n_fail} of {st.size} modes ended with a nonzero status "
                  f"({dict(zip(vals.tolist(), cnt.tolist()))}); no result reported")
-    total_steps = float(ns.sum())
+    attempted = float(ns.sum())
     kern_ms = {k: float(np.mean(v)) for k, v in kern.items()}
     value = nc / (ms_per_step * 1e-3)
     if args.arm == "adjoint":
-        # the tape holds every accepted state: the reverse launch processes
-        # (states - modes) steps; attempted steps bound the forward work
-        dom = "adjoint"
-        fl = total_steps * flops_per_reverse_step(cfg)
-        kname = "deb_adjoint_kernel"
+        accepted = _accepted(last["tape"], nc * nk)
+        dom, kname, units = "adjoint", "deb_adjoint_kernel", accepted
+        per_unit = flops.flops_per_reverse_step(cfg)
+        unit_name = "accepted steps (each recomputed from its checkpoint and reversed)"
     else:
-        dom = "forward"
-        fl = total_steps * flops_per_step(cfg)
-        kname = "deb_forward_kernel"
+        accepted = None
+        dom, kname, units = "forward", "deb_forward_kernel", attempted
+        per_unit = flops.flops_per_step(cfg)
+        unit_name = "attempted steps"
+    fl = units * per_unit
     achieved = fl / (kern_ms[dom] * 1e-3) / 1e12
+    workload = f"batched-{nc}cosmo-{nk}k"
     out = {
         "metric": METRIC,
         "value": value,
         "unit": "(P,dP/dtheta)/s" if args.arm != "forward" else "P(k)/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"batched-{nc}cosmo-{nk}k", "n_cosmo": nc, "num_k": nk, "lmax": 32, "lmaxnu": 17,
-                   "nqmax": 5, "rtol": 1e-5, "atol": 1e-8, "aexp_out": [1.0], "arm": args.arm,
+        "config": {"workload": workload, "n_cosmo": nc, "num_k": nk, "lmax": 32, "lmaxnu": 17, "nqmax": 5,
+                   "rtol": 1e-5, "atol": 1e-8, "aexp_out": [1.0], "arm": args.arm,
+                   "ckpt_every": args.ckpt_every if args.arm == "adjoint" else None,
                    "gradient": {"adjoint": "batched reverse launch (6 directions) + analytic n_s, ln10As",
                                 "fd": "central differences, 13 solves per cosmology in one launch",
                                 "forward": None}[args.arm]},
-        "roofline": {"bound": "fp64-vector", "achieved": achieved, "peak": PEAK_FP64, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64, "traffic": None, "kernel": kname,
-                     "kernel_ms": kern_ms[dom], "flops_per_launch": fl},
-        "kernel_ms": kern_ms, "modes": int(ns.size), "mode_steps": total_steps, "failed_modes": n_fail,
-        "k_modes_per_s": ns.size / (sum(kern_ms.values()) * 1e-3), "setup_s": setup_s,
+        "roofline": roofline(kname, workload, fl, kern_ms[dom], per_unit, units, unit_name),
+        "kernel_ms": kern_ms, "modes": int(ns.size), "attempted_steps": attempted, "accepted_steps": accepted,
+        "failed_modes": n_fail, "k_modes_per_s": ns.size / (ms_per_step * 1e-3), "setup_s": setup_s,
     }
     if args.arm == "adjoint":
-        out["tape_bytes"] = last["tape"]
+        out["tape_bytes"] = last["tape"].nbytes
         if args.plain_sample > 0:
             progress("timing the plain baseline")
-            out["plain_baseline"] = plain_baseline(cfg, th0[:args.plain_sample], nk, dev, value)
+            out["plain_baseline"] = plain_baseline(cfg, th0[:args.plain_sample], kmodes, dev, value)
     if not args.no_cpu_baseline:
         progress("timing the CPU baseline")
-        out["cpu_baseline"] = cpu_baseline(cfg, th0[:1], nk, args.cpu_sample_modes, args.arm)
+        out["cpu_baseline"] = cpu_baseline(cfg, th0[:1], kmodes, args.cpu_sample_modes, args.arm)
     print(json.dumps(out))
 
 
-def plain_baseline(cfg, thetas, nk, dev, adjoint_value):
+def roofline(kname, workload, fl, kernel_ms, per_unit, units, unit_name):
+    achieved = fl / (kernel_ms * 1e-3) / 1e12
+    r = {"bound": "mfma", "pipe": "fp64 (vector ALU; the MI355X FP64 peak is the same for the matrix cores)",
+         "achieved": achieved, "peak": PEAK_FP64, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64, "traffic": None,
+         "kernel": kname, "kernel_ms": kernel_ms, "flops_per_launch": fl, "flops_per_unit": per_unit,
+         "units_per_launch": units, "work_unit": unit_name}
+    tr = traffic_record(kname, workload)
+    if tr is not None:
+        r["traffic"] = tr["bytes_per_launch"]
+        r["traffic_source"] = tr["source"]
+    return r
+
+
+def run_loss_adjoint(args):
+    """Config 5: loss = sum_k w(k) ln P(k) per cosmology and its gradient."""
+    import torch
+    from discoeb_amd import flops, gradient, packing
+    from discoeb_amd.cosmology import SEED, loss_weights, sample_thetas
+
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    cfg = packing.make_config(lmaxg=50, lmaxgp=50, lmaxr=50, lmaxnu=17, nqmax=5, rtol=1e-5, atol=1e-8,
+                              max_steps=65536)
+    nc = args.n_cosmo or 256
+    nk = args.num_k or 512
+    th0 = sample_thetas(256, SEED)[:nc]
+    wk = torch.from_numpy(loss_weights(512, 256, SEED)[:nk]).to(dev)
+    kmodes = np.geomspace(1e-5, 20.0, nk)
+    t_setup = time.time()
+    _, inp, k_t, tau_t, tan = make_inputs(th0, cfg, dev, kmodes, with_tangents=True)
+    th_t = torch.from_numpy(th0).to(dev)
+    kk = torch.from_numpy(kmodes).to(dev)[None, :]
+    pref = prefactor(th_t, kk)
+    setup_s = time.time() - t_setup
+    progress(f"setup done ({setup_s:.1f} s): {nc} cosmologies x {nk} k, lmax 50")
+    last = {}
+    n_states = [None]
+
+    def step():
+        with Timer(stream) as tf:
+            y, st, ns, tape = gradient.solve_forward_tape(inp, k_t, tau_t, n_states=n_states[0],
+                                                          ckpt_every=args.ckpt_every)
+        dm = y[:, :, 0, 4]
+        loss = (wk[None, :] * torch.log(pref * dm**2)).sum(dim=1)
+        ybar = torch.zeros_like(y)
+        ybar[:, :, 0, 4] = wk[None, :] * 2.0 / dm  # d(w ln(pref dm^2)) / d dm
+        with Timer(stream) as ta:
+            g = gradient.solve_adjoint(inp, tan, k_t, tau_t, ybar, st, tape)
+        G = torch.empty((nc, 8), dtype=torch.float64, device=dev)
+        gs = g.sum(dim=1)
+        for j, d in enumerate(FD_PARAMS):
+            G[:, d] = gs[:, j]
+        G[:, 3] = (wk[None, :] * torch.log(kk / 0.05)).sum(dim=1)
+        G[:, 4] = wk.sum()
+        last.update(loss=loss, G=G, st=st, ns=ns, tape=tape, ev={"forward_tape": tf, "adjoint": ta})
+
+    for i in range(args.warmup):
+        tw = time.perf_counter()
+        step()
+        torch.cuda.synchronize()
+        progress(f"warmup step {i + 1}/{args.warmup} done ({time.perf_counter() - tw:.1f} s)")
+        n_states[0] = int(last["ns"].sum().item()) + nc * nk
+    kern = {}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step()
+        torch.cuda.synchronize()
+        progress(f"timed step {i + 1}/{args.steps} done")
+        for name, tm in last["ev"].items():
+            kern.setdefault(name, []).append(tm.ms())
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    st = last["st"].cpu().numpy()
+    n_fail = int((st != 0).sum())
+    if n_fail:
+        vals, cnt = np.unique(st[st != 0], return_counts=True)
+        sys.exit(f"bench: {n_fail} of {st.size} modes ended with a nonzero status "
+                 f"({dict(zip(vals.tolist(), cnt.tolist()))}); no result reported")
+    kern_ms = {k: float(np.mean(v)) for k, v in kern.items()}
+    accepted = _accepted(last["tape"], nc * nk)
+    fl = accepted * flops.flops_per_reverse_step(cfg)
+    workload = f"batched-adjoint-{nc}cosmo-{nk}k"
+    out = {
+        "metric": "(loss, dloss/dtheta) per sec, loss = sum_k w(k) ln P(k), lmax 50, rtol 1e-5",
+        "value": nc / (ms_per_step * 1e-3), "unit": "(loss,grad)/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": workload, "n_cosmo": nc, "num_k": nk, "kmax": 20.0, "lmax": 50, "lmaxnu": 17,
+                   "nqmax": 5, "rtol": 1e-5, "atol": 1e-8, "ckpt_every": args.ckpt_every,
+                   "device_budget_gib": gradient.TAPE_BUDGET / 2**30},
+        "roofline": roofline("deb_adjoint_kernel", workload, fl, kern_ms["adjoint"],
+                             flops.flops_per_reverse_step(cfg), accepted,
+                             "accepted steps (each recomputed from its checkpoint and reversed)"),
+        "kernel_ms": kern_ms, "modes": int(st.size), "attempted_steps": float(last["ns"].sum().item()),
+        "accepted_steps": accepted, "tape_bytes": last["tape"].nbytes,
+        "adjoint_workspace_bytes": gradient.adjoint_workspace_bytes(cfg, args.ckpt_every), "setup_s": setup_s,
+        "loss_mean": float(last["loss"].mean().item()),
+    }
+    print(json.dumps(out))
+
+
+def run_plain(args):
+    """Configs 1 and 2: one cosmology, forward solve."""
+    import torch
+    from discoeb_amd import flops, packing
+    from discoeb_amd.cosmology import base_theta
+    from discoeb_amd.perturbations import solve_forward
+
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    if args.workload == "plain-lcdm-64k":
+        cfg = packing.make_config(lmaxg=16, lmaxgp=16, lmaxr=16, lmaxnu=8, nqmax=3, rtol=1e-5, atol=1e-8,
+                                  max_steps=32768)
+        th = base_theta(w0=-1.0, wa=0.0)[None]
+        kmodes, aexp, massive = np.geomspace(1e-4, 1.0, args.num_k or 64), (1.0,), False
+    else:
+        cfg = packing.make_config(rtol=1e-5, atol=1e-8, max_steps=32768, **LMAX32)
+        th = base_theta()[None]
+        kmodes, aexp, massive = np.geomspace(1e-5, 10.0, args.num_k or 256), tuple(np.geomspace(0.01, 1, 16)), True
+    _, inp, k_t, tau_t, _ = make_inputs(th, cfg, dev, kmodes, aexp=aexp, massive_nu=massive)
+    res = {}
+    for i in range(args.warmup + args.steps):
+        with Timer(stream) as tm:
+            y, st, ns = solve_forward(inp, k_t, tau_t, throw=False)
+        torch.cuda.synchronize()
+        if i >= args.warmup:
+            res.setdefault("ms", []).append(tm.ms())
+    st = st.cpu().numpy()
+    if (st != 0).any():
+        sys.exit(f"bench: {int((st != 0).sum())} modes ended with a nonzero status; no result reported")
+    ms = float(np.mean(res["ms"]))
+    attempted = float(ns.sum().item())
+    fl = attempted * flops.flops_per_step(cfg)
+    out = {"metric": f"k-modes per sec, {args.workload}", "value": kmodes.size / (ms * 1e-3), "unit": "k-modes/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": args.workload, "num_k": int(kmodes.size), "nout": len(aexp),
+                      "lmax": cfg.lmaxg, "nqmax": cfg.nqmax, "massive_nu": massive, "rtol": 1e-5, "atol": 1e-8},
+           "p_k_per_s": 1.0 / (ms * 1e-3),
+           "roofline": roofline("deb_forward_kernel", args.workload, fl, ms, flops.flops_per_step(cfg), attempted,
+                                "attempted steps"),
+           "attempted_steps": attempted}
+    print(json.dumps(out))
+
+
+def plain_baseline(cfg, thetas, kmodes, dev, adjoint_value):
     """One cosmology per launch: forward + central differences of the six
     solve parameters (13 launches per cosmology), timed on a sample."""
     import torch
@@ -282,10 +461,10 @@ def plain_baseline(cfg, thetas, nk, dev, adjoint_value):
                 t = th[None].copy()
                 t[0, j] *= (1.0 + sgn * FD_REL)
                 ths.append(t)
-        prepared = [make_inputs(t, cfg, dev, nk, False) for t in ths]
+        prepared = [make_inputs(t, cfg, dev, kmodes) for t in ths]
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        outs = [solve_forward(inp, k_t, tau_t, throw=False) for (_, _, inp, k_t, tau_t, _) in prepared]
+        outs = [solve_forward(inp, k_t, tau_t, throw=False) for (_, inp, k_t, tau_t, _) in prepared]
         torch.cuda.synchronize()
         per.append(time.perf_counter() - t0)
         n_fail = sum(int((st != 0).sum().item()) for (_, st, _) in outs)
@@ -293,42 +472,50 @@ def plain_baseline(cfg, thetas, nk, dev, adjoint_value):
             sys.exit(f"bench: plain baseline: {n_fail} modes ended with a nonzero status; no result reported")
     sec = float(np.mean(per))
     return {"value": 1.0 / sec, "unit": "(P,dP/dtheta)/s", "launches_per_cosmology": 13,
-            "sample": f"{len(thetas)} cosmologies, each 13 launches of 1 cosmology x {nk} k",
+            "sample": f"{len(thetas)} cosmologies, each 13 launches of 1 cosmology x {kmodes.size} k",
             "batched_adjoint_over_plain": adjoint_value * sec}
 
 
-def cpu_baseline(cfg, theta, nk, nsample, arm):
+def cpu_baseline(cfg, theta, kmodes, nsample, arm):
     """The oracle (C++ restatement of the reference algorithm: dense LU Rodas5;
     gradients by forward-mode tangents through the frozen steps, like
-    jax.jacfwd) on this host's cores, on a bounded sample of modes of
-    cosmology 0.  The six tangent directions cost the same; one is timed."""
+    jax.jacfwd) on this host's cores, on a bounded sample: nsample of the k,
+    evenly spaced in index over the log grid, of cosmology 0, one mode per
+    thread.  Each mode's forward solve and its tangent replay are timed in its
+    own thread; the six tangent directions cost the same, one is timed.  The
+    cosmology's cost is the sample's summed mode time scaled to all k; value =
+    cores / cost (and 1 / cost for one thread)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     from discoeb_amd import packing, tangents
-    cores = os.cpu_count() or 1
-    kmodes = np.geomspace(1e-5, 10.0, nk)
-    sel = np.unique(np.linspace(0, nk - 1, nsample).astype(int))
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    nk = kmodes.size
+    sel = np.unique(np.linspace(0, nk - 1, nsample).round().astype(int))
     with np.errstate(all="ignore"):
         param, dcos, dtab, dtau = tangents.input_tangents(theta, [1.0])
     prob = oracle.Problem(param, cfg, b=0)
     tau_out = packing.tau_of_aexp(param, [1.0])[0].real
     thr = min(cores, sel.size)
     t = time.perf_counter()
-    prob.evolve(kmodes[sel], tau_out, nthreads=thr)
-    t_fwd = time.perf_counter() - t
+    rf = prob.evolve(kmodes[sel], tau_out, nthreads=thr)
+    wall = time.perf_counter() - t
+    fwd = rf["seconds"]
     if arm == "forward":
-        wall, what = t_fwd, "forward solve"
+        per_mode = fwd
+        what = f"forward solves: {fwd.sum():.1f} mode-s"
     else:
         t = time.perf_counter()
-        prob.evolve_tangent(kmodes[sel], tau_out, dtau[0, 0], dcos[0, 0], dtab[0, 0], nthreads=thr)
-        t_tan = time.perf_counter() - t  # a forward solve plus one tangent replay
-        wall = t_fwd + len(tangents.SOLVE_DIRS) * max(t_tan - t_fwd, 0.0)
-        what = f"forward {t_fwd:.1f}s + 6 x tangent replay {max(t_tan - t_fwd, 0.0):.1f}s"
-    per_cosmo = wall * thr / sel.size * nk / cores
-    return {"value": 1.0 / per_cosmo, "unit": "(P,dP/dtheta)/s" if arm != "forward" else "P(k)/s",
-            "cores": cores, "kind": "port",
-            "sample": f"{sel.size} of {nk} k (log-spaced subset) of cosmology 0 on {thr} threads: {what}; "
-                      f"scaled to {nk} k on {cores} cores"}
+        rt = prob.evolve_tangent(kmodes[sel], tau_out, dtau[0, 0], dcos[0, 0], dtab[0, 0], nthreads=thr)
+        wall += time.perf_counter() - t
+        rep = np.maximum(rt["seconds"] - fwd, 0.0)  # a tangent solve is a forward solve plus the replay
+        per_mode = fwd + len(tangents.SOLVE_DIRS) * rep
+        what = f"forward {fwd.sum():.1f} mode-s + 6 x tangent replay {rep.sum():.1f} mode-s"
+    cost = per_mode.sum() * nk / sel.size  # seconds of one core for the whole cosmology
+    return {"value": cores / cost, "unit": "(P,dP/dtheta)/s" if arm != "forward" else "P(k)/s",
+            "cores": cores, "kind": "port", "value_1_thread": 1.0 / cost,
+            "sample": f"{sel.size} of {nk} k (every {max(nk // sel.size, 1)}th of the log grid) of cosmology 0, "
+                      f"one mode per thread on {thr} threads ({wall:.1f} s wall): {what}; "
+                      f"scaled to {nk} k"}
 
 
 if __name__ == "__main__":

@@ -758,18 +758,18 @@ __device__ __forceinline__ TD seed_td(double v, double d0) {
   return r;
 }
 
-// the forward step from (t0, y0) again: factors into m, stage vectors into Kl
-// (inlined, with its factorisation, right-hand sides and solves as calls: the
-// whole step as one call is large enough to need long branches, which must
-// stay out of non-kernel functions)
+// one forward step from (t0, y0) to y1, its eight stage vectors stored in
+// K[8][NS][64] (inlined, with its factorisation, right-hand sides and solves
+// as calls: the whole step as one call is large enough to need long branches,
+// which must stay out of non-kernel functions)
 template <int NS>
-__device__ __forceinline__ void recompute_step(Mode<NS> &m, Ctx &cx, double t0, double h, const double (&y0)[NS],
-                                               double *Kl) {
+__device__ __forceinline__ void step_store(Mode<NS> &m, Ctx &cx, double t0, double h, const double (&y0)[NS],
+                                           double (&y1)[NS], double *K) {
   const int lane = lane_id();
-  double y1[NS], err[NS];
+  double err[NS];
   auto store = [&](int i, const double (&k)[NS]) {
 #pragma unroll
-    for (int s = 0; s < NS; ++s) Kl[(i * NS + s) * WAVE + lane] = k[s];
+    for (int s = 0; s < NS; ++s) K[(i * NS + s) * WAVE + lane] = k[s];
   };
   rodas5_step_s<NS, decltype(store), true>(m, cx, t0, h, y0, y1, err, store);
 }
@@ -777,15 +777,17 @@ __device__ __forceinline__ void recompute_step(Mode<NS> &m, Ctx &cx, double t0, 
 // ---------------------------------------------------------------------------
 // reverse of one Rodas5 step.  In: ybar = dL/dy1.  Out: ybar = dL/dy0,
 // tbar += dL/dt0 (h fixed), hbar += dL/dh, thb[j] += dL/dtheta_j (wave-uniform).
-// Kl / KBl: per-wave LDS scratch of 8 and 7 stage vectors.
+// K: the step's eight stage vectors ([8][NS][64], written by step_store);
+// KBl: per-wave LDS scratch of 7 stage vectors.  The factorisation of W at
+// (t0, y0) is recomputed into m.
 template <int NS>
 __device__ __forceinline__ void rodas5_step_vjp(Mode<NS> &m, Ctx &cx, double t0, double h,
                                                 const double (&y0)[NS], double (&ybar)[NS], double (&thb)[MAXDIR],
-                                                double &tbar, double &hbar, double *Kl, double *KBl) {
+                                                double &tbar, double &hbar, const double *K, double *KBl) {
   const int lane = lane_id();
-  auto Kp = [&](int i, int s) -> double & { return Kl[(i * NS + s) * WAVE + lane]; };
+  auto Kp = [&](int i, int s) -> double { return K[(i * NS + s) * WAVE + lane]; };
   auto KBp = [&](int i, int s) -> double & { return KBl[(i * NS + s) * WAVE + lane]; };
-  recompute_step<NS>(m, cx, t0, h, y0, Kl);
+  factor_call<NS>(m, cx, t0, y0, h);
   double tw[NS];
   trunc_weights<NS>(m.rd, tw);
   const double it0 = 1.0 / t0;
@@ -1095,9 +1097,30 @@ struct AdjArgs {
   const int32_t *status;  // [n_cosmo][nk] of the taped forward solve
   double *grad;           // [n_cosmo][nk][ndir]
   TapeView tape;
+  double *rings;          // [n_rings][ring_doubles(NS, C)] reverse-launch rings
+  int32_t *ring_busy;     // [n_rings] 1 while a wave owns the ring
+  int n_rings;
   int n_in;               // state entries read by the output conversion
   int32_t in_idx[WAVE];   // their reference indices, one per lane
 };
+
+// reference indices of the state entries read by the output conversion, one
+// per lane (host side: fills AdjArgs::in_idx)
+inline int output_inputs(const deb_config_t &c, int32_t *idx) {
+  const int lg = c.lmaxg, lgp = c.lmaxgp, lr = c.lmaxr, nq = c.nqmax;
+  const int n = 12 + lg + lgp + lr + nq * (c.lmaxnu + 1);
+  const int ir = 9 + lg + lgp, iq0 = 10 + lg + lgp + lr;
+  int m = 0;
+  for (int i : {0, 2, 3, 4, 5, 6, 7, 8}) idx[m++] = i;
+  idx[m++] = ir;
+  idx[m++] = ir + 1;
+  for (int i = 0; i < nq; ++i) idx[m++] = iq0 + i;
+  for (int i = 0; i < nq; ++i) idx[m++] = iq0 + nq + i;
+  idx[m++] = n - 2;
+  idx[m++] = n - 1;
+  for (int i = m; i < WAVE; ++i) idx[i] = -1;
+  return m;
+}
 
 __device__ __forceinline__ void lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
@@ -1148,9 +1171,11 @@ __device__ __forceinline__ void output_vjp(const Params<NS> &P, Ctx &cx, const A
   lds_sync();
 }
 
-// one (cosmology, k) item of the gradient kernel
+// one (cosmology, k) item of the gradient kernel.  ring: this wave's ring
+// (ring_doubles(NS, C) doubles of device memory, owned for the item's life);
+// KBl, ls, lg: per-wave LDS scratch.
 template <int NS>
-__device__ __forceinline__ void adjoint_mode(const Params<NS> &P, const AdjArgs &A, int ci, int ki, double *Kl,
+__device__ __forceinline__ void adjoint_mode(const Params<NS> &P, const AdjArgs &A, int ci, int ki, double *ring,
                                              double *KBl, double *ls, double *lg) {
   const int lane = lane_id();
   const long long item = (long long)ci * P.nk + ki;
@@ -1176,87 +1201,108 @@ __device__ __forceinline__ void adjoint_mode(const Params<NS> &P, const AdjArgs 
     if (tout[i] < tout[imin]) imin = i;
     if (tout[i] > tout[imax]) imax = i;
   }
-  // tape reader: states r = N .. 0 of this mode, last chunk first
+  // the tape: N accepted steps in segments of C, the last segment's chunk
+  // first (a chunk that holds only the final state, when C divides N, is skipped)
   const TapeView &tv = A.tape;
+  const int C = tv.ckpt;
   const int N = tv.count[item] - 1;
+  auto chunk_t = [&](long long c) { return reinterpret_cast<const double *>(tv.pool + c * tv.chunk_bytes + 64); };
+  auto chunk_y = [&](long long c) { return reinterpret_cast<const double *>(tv.pool + c * tv.chunk_bytes + tv.state_off); };
+  auto chunk_prev = [&](long long c) { return *reinterpret_cast<const long long *>(tv.pool + c * tv.chunk_bytes); };
   long long chunk = tv.last[item];
-  int chunk_ord = N / TAPE_CH;
-  auto load = [&](int r, double &t, double (&y)[NS]) {
-    while (r / TAPE_CH < chunk_ord) {
-      chunk = *reinterpret_cast<const long long *>(tv.pool + chunk * tv.chunk_bytes);
-      --chunk_ord;
-    }
-    const double *tt = reinterpret_cast<const double *>(tv.pool + chunk * tv.chunk_bytes + 64);
-    const double *yy = tt + TAPE_CH;
-    const int slot = r % TAPE_CH;
-    t = tt[slot];
-#pragma unroll
-    for (int s = 0; s < NS; ++s) y[s] = yy[(slot * NS + s) * WAVE + lane];
-  };
-  double ynext[NS], ycur[NS], yb[NS], pend[NS];
-  double tnext, tcur, t0 = 0.0;
-  load(N, tnext, ynext);
-  {
-    double tz;
-    double yz[NS];
-    // t0 is the first state's time
-    long long save_chunk = chunk;
-    int save_ord = chunk_ord;
-    load(0, tz, yz);
-    t0 = tz;
-    chunk = save_chunk;
-    chunk_ord = save_ord;
+  int seg = N / C;
+  if (N % C == 0) {
+    chunk = chunk_prev(chunk);
+    --seg;
   }
+  double t0 = 0.0;  // the first state's time
+  {
+    long long c = chunk;
+    for (int s2 = seg; s2 > 0; --s2) c = chunk_prev(c);
+    t0 = chunk_t(c)[0];
+  }
+  auto ring_y = [&](int j) { return ring + (long long)j * NS * WAVE; };
+  auto ring_k = [&](int j) { return ring + ((long long)(C + 1) + 8LL * j) * NS * WAVE; };
   const double T1 = tout[imax];
   const double span = T1 - t0;
   double thb[MAXDIR];
 #pragma unroll
   for (int j = 0; j < MAXDIR; ++j) thb[j] = 0.0;
   double Tb0 = 0.0, Tb1 = 0.0;
+  double yb[NS], pend[NS];
 #pragma unroll
   for (int s = 0; s < NS; ++s) yb[s] = 0.0;
   const double *wbase = A.ybar + item * (long long)P.nout * DEB_NOUT_FIELDS;
   int iout = P.nout - 1;
-  for (int n = N - 1; n >= 0; --n) {
-    load(n, tcur, ycur);
-    const double h = tnext - tcur;
-    double tb = 0.0, hb = 0.0;
+  for (; seg >= 0; --seg, chunk = chunk_prev(chunk)) {
+    const int nL = min(C, N - seg * C);
+    const double *tt = chunk_t(chunk);
+    // forward again over the segment from its checkpoint: states and stage
+    // vectors into the ring
+    {
+      double y[NS], y1[NS];
+      const double *yc = chunk_y(chunk);
 #pragma unroll
-    for (int s = 0; s < NS; ++s) pend[s] = 0.0;
-    // outputs interpolated inside this step (t_n < tau_o <= t_{n+1}; step 0 also
-    // takes tau_o <= t_1)
-    while (iout >= 0 && (tout[iout] > tcur || n == 0)) {
-      const double th = (tout[iout] - tcur) / h;
-      double yo[NS], ybo[NS];
+      for (int s = 0; s < NS; ++s) y[s] = yc[s * WAVE + lane];
+      for (int j = 0; j < nL; ++j) {
+        double *ry = ring_y(j);
 #pragma unroll
-      for (int s = 0; s < NS; ++s) yo[s] = ycur[s] + th * (ynext[s] - ycur[s]);
-      output_vjp<NS>(P, cx, A, m.rd, yo, wbase + (long long)iout * DEB_NOUT_FIELDS, ls, lg, ybo, thb);
-      double pth = 0.0;
+        for (int s = 0; s < NS; ++s) ry[s * WAVE + lane] = y[s];
+        step_store<NS>(m, cx, tt[j], tt[j + 1] - tt[j], y, y1, ring_k(j));
 #pragma unroll
-      for (int s = 0; s < NS; ++s) {
-        yb[s] += th * ybo[s];
-        pend[s] += (1.0 - th) * ybo[s];
-        pth += ybo[s] * (ynext[s] - ycur[s]);
+        for (int s = 0; s < NS; ++s) y[s] = y1[s];
       }
-      const double thbar = wave_sum1(pth);
-      const double taub = thbar / h;
+      double *ry = ring_y(nL);
 #pragma unroll
-      for (int j = 0; j < MAXDIR; ++j)
-        if (j < A.ndir) thb[j] += taub * dtout[j * P.nout + iout];
-      tb += -thbar / h;
-      hb += -thbar * th / h;
-      --iout;
+      for (int s = 0; s < NS; ++s) ry[s * WAVE + lane] = y[s];
     }
-    rodas5_step_vjp<NS>(m, cx, tcur, h, ycur, yb, thb, tb, hb, Kl, KBl);
+    for (int j = nL - 1; j >= 0; --j) {
+      const int n = seg * C + j;
+      double ycur[NS], ynext[NS];
+      {
+        const double *r0 = ring_y(j), *r1 = ring_y(j + 1);
 #pragma unroll
-    for (int s = 0; s < NS; ++s) {
-      yb[s] += pend[s];
-      ynext[s] = ycur[s];
+        for (int s = 0; s < NS; ++s) {
+          ycur[s] = r0[s * WAVE + lane];
+          ynext[s] = r1[s * WAVE + lane];
+        }
+      }
+      const double tcur = tt[j];
+      const double h = tt[j + 1] - tcur;
+      double tb = 0.0, hb = 0.0;
+#pragma unroll
+      for (int s = 0; s < NS; ++s) pend[s] = 0.0;
+      // outputs interpolated inside this step (t_n < tau_o <= t_{n+1}; step 0
+      // also takes tau_o <= t_1)
+      while (iout >= 0 && (tout[iout] > tcur || n == 0)) {
+        const double th = (tout[iout] - tcur) / h;
+        double yo[NS], ybo[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) yo[s] = ycur[s] + th * (ynext[s] - ycur[s]);
+        output_vjp<NS>(P, cx, A, m.rd, yo, wbase + (long long)iout * DEB_NOUT_FIELDS, ls, lg, ybo, thb);
+        double pth = 0.0;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+          yb[s] += th * ybo[s];
+          pend[s] += (1.0 - th) * ybo[s];
+          pth += ybo[s] * (ynext[s] - ycur[s]);
+        }
+        const double thbar = wave_sum1(pth);
+        const double taub = thbar / h;
+#pragma unroll
+        for (int jj = 0; jj < MAXDIR; ++jj)
+          if (jj < A.ndir) thb[jj] += taub * dtout[jj * P.nout + iout];
+        tb += -thbar / h;
+        hb += -thbar * th / h;
+        --iout;
+      }
+      rodas5_step_vjp<NS>(m, cx, tcur, h, ycur, yb, thb, tb, hb, ring_k(j), KBl);
+#pragma unroll
+      for (int s = 0; s < NS; ++s) yb[s] += pend[s];
+      const double sn = (tcur - t0) / span, dsn = h / span;
+      Tb0 += tb * (1.0 - sn) - hb * dsn;
+      Tb1 += tb * sn + hb * dsn;
     }
-    const double sn = (tcur - t0) / span, dsn = h / span;
-    Tb0 += tb * (1.0 - sn) - hb * dsn;
-    Tb1 += tb * sn + hb * dsn;
-    tnext = tcur;
   }
   // initial conditions y(t0) = IC(t0)
   {

@@ -11,6 +11,7 @@
 //                            evaluation and of one Rodas5 step
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -19,9 +20,14 @@
 
 namespace deb {
 
+// One workgroup (one wavefront) per (cosmology, k) item, largest k first.
+// Each wave first takes a free ring (the state and stage-vector buffer of the
+// segment it reverses) by a bounded atomic search from its own index: the host
+// provides twice as many rings as waves can be resident, so the search ends
+// within a few probes; a wave that finds none after 4 passes gives NaN
+// gradients instead of waiting.  The ring is released when the item is done.
 template <int NS>
 __global__ void __launch_bounds__(WAVE) deb_adjoint_kernel(const Params<NS> P, const AdjArgs A) {
-  __shared__ double Kl[8 * NS * WAVE];
   __shared__ double KBl[7 * NS * WAVE];
   __shared__ double ls[NS * WAVE];
   __shared__ double lg[NS * WAVE];
@@ -29,7 +35,32 @@ __global__ void __launch_bounds__(WAVE) deb_adjoint_kernel(const Params<NS> P, c
   if (item >= P.n_cosmo * P.nk) return;
   const int ki = P.nk - 1 - item / P.n_cosmo;
   const int ci = item % P.n_cosmo;
-  adjoint_mode<NS>(P, A, ci, ki, Kl, KBl, ls, lg);
+  const int lane = lane_id();
+  int ring = -1;
+  if (lane == 0) {
+    const int start = item % A.n_rings;
+    for (int t = 0; t < 4 * A.n_rings; ++t) {
+      const int r = (start + t) % A.n_rings;
+      if (atomicCAS(&A.ring_busy[r], 0, 1) == 0) {
+        ring = r;
+        break;
+      }
+    }
+  }
+  ring = __builtin_amdgcn_readfirstlane(ring);
+  if (ring < 0) {
+    if (lane < A.ndir) A.grad[((long long)ci * P.nk + ki) * A.ndir + lane] = __builtin_nan("");
+    return;
+  }
+  adjoint_mode<NS>(P, A, ci, ki, A.rings + (long long)ring * ring_doubles(NS, A.tape.ckpt), KBl, ls, lg);
+  __threadfence();
+  if (lane == 0) atomicExch(&A.ring_busy[ring], 0);
+}
+
+// marks every ring free (a kernel rather than a memset, so that a caller may
+// capture the launch sequence in a graph)
+__global__ void deb_ring_reset_kernel(int32_t *busy, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) busy[i] = 0;
 }
 
 struct VjpHook {
@@ -120,6 +151,8 @@ __global__ void __launch_bounds__(WAVE) deb_step_vjp_kernel(const Params<NS> P, 
 #pragma unroll
   for (int j = 0; j < MAXDIR; ++j) thb[j] = 0.0;
   double tb = 0.0, hb = 0.0;
+  double y1[NS];
+  step_store<NS>(m, cx, t0, h, y, y1, Kl);
   rodas5_step_vjp<NS>(m, cx, t0, h, y, yb, thb, tb, hb, Kl, KBl);
   double *yo = H.ybar + (long long)item * P.n;
 #pragma unroll
@@ -137,29 +170,41 @@ __global__ void __launch_bounds__(WAVE) deb_step_vjp_kernel(const Params<NS> P, 
 // ---------------------------------------------------------------------------
 // host side
 
-// reference indices of the state entries read by the output conversion
-inline int output_inputs(const deb_config_t &c, int32_t *idx) {
-  const int lg = c.lmaxg, lgp = c.lmaxgp, lr = c.lmaxr, nq = c.nqmax;
-  const int n = 12 + lg + lgp + lr + nq * (c.lmaxnu + 1);
-  const int ir = 9 + lg + lgp, iq0 = 10 + lg + lgp + lr;
-  int m = 0;
-  for (int i : {0, 2, 3, 4, 5, 6, 7, 8}) idx[m++] = i;
-  idx[m++] = ir;
-  idx[m++] = ir + 1;
-  for (int i = 0; i < nq; ++i) idx[m++] = iq0 + i;
-  for (int i = 0; i < nq; ++i) idx[m++] = iq0 + nq + i;
-  idx[m++] = n - 2;
-  idx[m++] = n - 1;
-  for (int i = m; i < WAVE; ++i) idx[i] = -1;
-  return m;
+// rings of the reverse launch: twice the waves the device can hold at the
+// kernel's occupancy
+template <int NS>
+int adjoint_rings(int &n_rings) {
+  int dev = 0, ncu = 0, per_cu = 0;
+  if (int r = hip_check(hipGetDevice(&dev), "hipGetDevice")) return r;
+  if (int r = hip_check(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev), "CU count"))
+    return r;
+  if (int r = hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                            &per_cu, reinterpret_cast<const void *>(deb_adjoint_kernel<NS>), WAVE, 0),
+                        "occupancy query"))
+    return r;
+  n_rings = 2 * std::max(per_cu, 1) * std::max(ncu, 1);
+  return 0;
+}
+
+inline long long adjoint_ws_bytes(int ns, int ckpt, int n_rings) {
+  const long long flags = ((long long)n_rings * 4 + 255) / 256 * 256;
+  return flags + (long long)n_rings * ring_doubles(ns, ckpt) * 8;
+}
+
+template <int NS>
+int adjoint_workspace(int ckpt, size_t *bytes) {
+  int n_rings = 0;
+  if (int r = adjoint_rings<NS>(n_rings)) return r;
+  *bytes = (size_t)adjoint_ws_bytes(NS, ckpt, n_rings);
+  return 0;
 }
 
 template <int NS>
 int run_adjoint(const deb_config_t *cfg, const Layout &L, int n_cosmo, const deb_cosmo_t *cosmo, const double *tables,
                 int64_t stride, const int32_t *desc, const double *q, const double *w, int nk, const double *kmodes,
                 int nout, const double *tau_out, int ndir, const double *dcos, const double *dtab,
-                const double *dtau, const double *ybar, const int32_t *status, double *grad, const void *tape,
-                size_t tape_bytes, hipStream_t st) {
+                const double *dtau, const double *ybar, const int32_t *status, double *grad, int ckpt,
+                const void *tape, size_t tape_bytes, void *ws, size_t ws_bytes, hipStream_t st) {
   Params<NS> P;
   if (int r = fill_params<NS>(P, cfg, L, n_cosmo, cosmo, tables, stride, desc, q, w, nk, kmodes, nout, tau_out))
     return r;
@@ -172,8 +217,17 @@ int run_adjoint(const deb_config_t *cfg, const Layout &L, int n_cosmo, const deb
   A.ybar = ybar;
   A.status = status;
   A.grad = grad;
-  if (int r = tape_view(A.tape, const_cast<void *>(tape), tape_bytes, (long long)n_cosmo * nk, NS)) return r;
+  if (int r = tape_view(A.tape, const_cast<void *>(tape), tape_bytes, (long long)n_cosmo * nk, NS, ckpt)) return r;
+  int n_rings = 0;
+  if (int r = adjoint_rings<NS>(n_rings)) return r;
+  const long long need = adjoint_ws_bytes(NS, ckpt, n_rings);
+  if (!ws || (long long)ws_bytes < need) return fail(DEB_E_ARG, "workspace smaller than deb_adjoint_workspace_bytes");
+  A.ring_busy = static_cast<int32_t *>(ws);
+  A.rings = reinterpret_cast<double *>(static_cast<char *>(ws) + (need - (long long)n_rings * ring_doubles(NS, ckpt) * 8));
+  A.n_rings = n_rings;
   A.n_in = output_inputs(*cfg, A.in_idx);
+  hipLaunchKernelGGL(deb_ring_reset_kernel, dim3((n_rings + 255) / 256), dim3(256), 0, st, A.ring_busy, n_rings);
+  if (int r = hip_check(hipGetLastError(), "ring reset launch")) return r;
   hipLaunchKernelGGL(deb_adjoint_kernel<NS>, dim3((unsigned)((int64_t)n_cosmo * nk)), dim3(WAVE), 0, st, P, A);
   return hip_check(hipGetLastError(), "adjoint kernel launch");
 }
@@ -201,21 +255,32 @@ using namespace deb;
 
 extern "C" {
 
-int deb_tape_bytes(const deb_config_t *cfg, int n_cosmo, int nk, int64_t n_states, size_t *bytes) {
+int deb_tape_bytes(const deb_config_t *cfg, int n_cosmo, int nk, int64_t n_states, int ckpt_every, size_t *bytes) {
   if (!cfg || !bytes || n_cosmo < 1 || nk < 1 || n_states < 0) return fail(DEB_E_ARG, "invalid argument");
+  if (ckpt_every < 1 || ckpt_every > TAPE_MAX_CKPT) return fail(DEB_E_ARG, "checkpoint interval outside [1, 4096]");
   Layout L;
   if (int r = build_layout(*cfg, L)) return r;
   const long long n_modes = (long long)n_cosmo * nk;
-  const long long chunks = n_states / TAPE_CH + n_modes + 1;
-  *bytes = (size_t)(tape_pool_off(n_modes) + chunks * tape_chunk_bytes(L.ns));
+  // one chunk per started segment of every mode: at most n_states / C + n_modes
+  const long long chunks = n_states / ckpt_every + n_modes + 1;
+  *bytes = (size_t)(tape_pool_off(n_modes) + chunks * tape_chunk_bytes(L.ns, ckpt_every));
   return 0;
+}
+
+int deb_adjoint_workspace_bytes(const deb_config_t *cfg, int ckpt_every, size_t *bytes) {
+  if (!cfg || !bytes) return fail(DEB_E_ARG, "invalid argument");
+  if (ckpt_every < 1 || ckpt_every > TAPE_MAX_CKPT) return fail(DEB_E_ARG, "checkpoint interval outside [1, 4096]");
+  Layout L;
+  if (int r = build_layout(*cfg, L)) return r;
+  DEB_DISPATCH(L.ns, (adjoint_workspace<NS>(ckpt_every, bytes)))
 }
 
 int deb_solve_adjoint(const deb_config_t *cfg, int n_cosmo, const deb_cosmo_t *cosmo, const double *tables,
                       int64_t table_stride, const int32_t *desc, const double *q, const double *w, int nk,
                       const double *kmodes, int nout, const double *tau_out, int ndir, const double *dcosmo,
                       const double *dtables, const double *dtau_out, const double *ybar, const int32_t *status,
-                      double *grad, const void *tape, size_t tape_bytes, void *stream) {
+                      double *grad, int ckpt_every, const void *tape, size_t tape_bytes, void *workspace,
+                      size_t workspace_bytes, void *stream) {
   if (int r = check_common(cfg, n_cosmo, cosmo, tables, desc, q, w)) return r;
   if (int r = check_dirs(ndir, dcosmo, dtables)) return r;
   if (nk < 1 || nout < 1 || !kmodes || !tau_out || !dtau_out || !ybar || !status || !grad)
@@ -226,8 +291,8 @@ int deb_solve_adjoint(const deb_config_t *cfg, int n_cosmo, const deb_cosmo_t *c
   if (int r = build_layout(*cfg, L)) return r;
   hipStream_t st = (hipStream_t)stream;
   DEB_DISPATCH(L.ns, (run_adjoint<NS>(cfg, L, n_cosmo, cosmo, tables, table_stride, desc, q, w, nk, kmodes, nout,
-                                      tau_out, ndir, dcosmo, dtables, dtau_out, ybar, status, grad, tape, tape_bytes,
-                                      st)))
+                                      tau_out, ndir, dcosmo, dtables, dtau_out, ybar, status, grad, ckpt_every, tape,
+                                      tape_bytes, workspace, workspace_bytes, st)))
 }
 
 int deb_rhs_vjp(const deb_config_t *cfg, const deb_cosmo_t *cosmo, const double *tables, int64_t table_stride,

@@ -377,24 +377,30 @@ __device__ __forceinline__ double wave_sum1(double x) {
 }
 
 // ----------------------------------------------------------------------------
-// checkpoint tape of the gradient path: every accepted state of every mode.
-// Chunks of TAPE_CH states are handed out from one pool by an atomic counter
-// and linked backwards, so a mode's states are read back last-to-first.
-//   chunk: [int64 prev chunk (-1: none)][pad to 64 B][double t[CH]][double y[CH][NS][64]]
-constexpr int TAPE_CH = 16;
+// checkpoint tape of the gradient path.  The accepted states n = 0..N of a mode
+// are grouped in segments of C consecutive steps: segment s keeps the
+// checkpoint state y_{sC} and the times t_{sC} .. t_{sC+C}.  One chunk per
+// segment, handed out from one pool by an atomic counter and linked backwards
+// (a mode's segments are read back last-to-first).  The reverse launch
+// recomputes each segment forward from its checkpoint.
+//   chunk:  [int64 prev chunk (-1: none)][pad][double t[C + 1]][pad to 256 B][double y[NS][64]]
+//   buffer: [header 256 B: uint64 chunks handed out][int64 last[n_modes]][int32 count[n_modes]][pool]
+constexpr int TAPE_MAX_CKPT = 4096;
 
-__host__ __device__ inline long long tape_chunk_bytes(int ns) {
-  long long b = 64 + 8LL * TAPE_CH + 8LL * TAPE_CH * ns * WAVE;
-  return (b + 255) / 256 * 256;
+__host__ __device__ inline long long tape_state_off(int ckpt) { return (64 + 8LL * (ckpt + 1) + 255) / 256 * 256; }
+__host__ __device__ inline long long tape_chunk_bytes(int ns, int ckpt) {
+  return tape_state_off(ckpt) + 8LL * ns * WAVE;
 }
 
 struct TapeView {
   unsigned long long *counter;  // chunks handed out so far
   long long capacity;           // chunks in the pool
-  long long *last;              // [n_modes] last chunk of a mode
+  long long *last;              // [n_modes] chunk of a mode's last segment
   int32_t *count;               // [n_modes] states recorded (-1: pool exhausted)
   char *pool;
   long long chunk_bytes;
+  long long state_off;          // byte offset of the checkpoint state in a chunk
+  int ckpt;                     // steps per segment C
 };
 
 // byte layout of a tape buffer for n_modes modes: header, per-mode arrays, pool
@@ -403,6 +409,13 @@ __host__ __device__ inline long long tape_count_off(long long n_modes) { return 
 __host__ __device__ inline long long tape_pool_off(long long n_modes) {
   long long b = 256 + 12 * n_modes;
   return (b + 255) / 256 * 256;
+}
+
+// Per-wave ring of the reverse launch: the states y_{sC} .. y_{sC+L} and the
+// eight stage vectors of every step of the segment being reversed, each a
+// lane-contiguous [NS][64] block.
+__host__ __device__ inline long long ring_doubles(int ns, int ckpt) {
+  return (long long)ns * WAVE * (9LL * ckpt + 1);
 }
 
 // ----------------------------------------------------------------------------

@@ -213,7 +213,8 @@ __device__ __forceinline__ unsigned long long readfirst_u64(unsigned long long x
   return ((unsigned long long)hi << 32) | lo;
 }
 
-// appends (t, y) states of one mode to the checkpoint tape
+// records the accepted states of one mode on the checkpoint tape: every
+// state's time, and the state itself at the start of each segment of C steps
 template <int NS>
 struct TapeWriter {
   TapeView tv;
@@ -221,11 +222,16 @@ struct TapeWriter {
   long long cur;
   int cnt;
   bool full;
+  __device__ __forceinline__ double *times(long long chunk) const {
+    return reinterpret_cast<double *>(tv.pool + chunk * tv.chunk_bytes + 64);
+  }
   __device__ __forceinline__ void put(double t, const double (&y)[NS]) {
     if (full) return;
     const int lane = lane_id();
-    const int slot = cnt % TAPE_CH;
-    if (slot == 0) {
+    const int j = cnt % tv.ckpt;
+    if (j == 0) {
+      // state cnt ends the previous segment and starts a new one
+      if (cnt > 0 && lane == 0) times(cur)[tv.ckpt] = t;
       unsigned long long c = 0;
       if (lane == 0) c = atomicAdd(tv.counter, 1ull);
       c = readfirst_u64(c);
@@ -236,12 +242,11 @@ struct TapeWriter {
       char *chunk = tv.pool + (long long)c * tv.chunk_bytes;
       if (lane == 0) *reinterpret_cast<long long *>(chunk) = cur;
       cur = (long long)c;
-    }
-    double *tt = reinterpret_cast<double *>(tv.pool + cur * tv.chunk_bytes + 64);
-    double *yy = tt + TAPE_CH;
-    if (lane == 0) tt[slot] = t;
+      double *yy = reinterpret_cast<double *>(chunk + tv.state_off);
 #pragma unroll
-    for (int s = 0; s < NS; ++s) yy[(slot * NS + s) * WAVE + lane] = y[s];
+      for (int s = 0; s < NS; ++s) yy[s * WAVE + lane] = y[s];
+    }
+    if (lane == 0) times(cur)[j] = t;
     ++cnt;
   }
   __device__ __forceinline__ void finish() {
@@ -251,6 +256,25 @@ struct TapeWriter {
     }
   }
 };
+
+// diffrax replaces a NaN error by inf: a non-finite norm rejects the step and
+// shrinks it by factormin
+__device__ __forceinline__ double norm_or_inf(double nrm) { return nrm <= 1.7976931348623157e308 ? nrm : INFINITY; }
+
+// true on every lane if any lane holds a non-finite state entry
+template <int NS>
+__device__ __forceinline__ bool any_nonfinite(const double (&y)[NS]) {
+  bool bad = false;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) bad = bad || !isfinite(y[s]);
+  return __ballot(bad) != 0;
+}
+
+// outputs a failed mode never reached hold inf (diffrax fills unreached
+// SaveAt entries with inf)
+__device__ __forceinline__ void fill_unreached(double *dst, int iout, int nout, int width) {
+  for (int i = iout * width + lane_id(); i < nout * width; i += WAVE) dst[i] = INFINITY;
+}
 
 __device__ __forceinline__ int mode_levels(const deb_config_t &cf, int nq) {
   int maxlen = cf.lmaxg + 2;
@@ -318,18 +342,11 @@ __device__ __forceinline__ void integrate_mode(const Params<NS> &P, int ci, int 
     }
     const double dt = tnext - tprev;
     rodas5_step<NS>(m, cx, tprev, dt, y, y1, err);
-    double e2 = 0.0;
-    {
-      double y0s = y[0], y1s = y1[0];
-      double sc = err[0] / (cf.atol + fmax(fabs(y0s), fabs(y1s)) * cf.rtol);
-      double x = sc * ew;
-      e2 = x * x;
-    }
-    const double nrm = sqrt(wave_sum1(e2) / 6.0);
-    if (!isfinite(nrm)) {
-      status = DEB_STATUS_NONFINITE;
-      break;
-    }
+    // filtered, weighted RMS of the scaled error: lanes outside the filter
+    // never enter it, so a non-finite unfiltered entry cannot poison it
+    const double sc = err[0] / (cf.atol + fmax(fabs(y[0]), fabs(y1[0])) * cf.rtol);
+    const double x = ew != 0.0 ? sc * ew : 0.0;
+    const double nrm = norm_or_inf(sqrt(wave_sum1(x * x) / 6.0));
     const bool keep = nrm < 1.0;
     double inv = 1.0 / nrm;
     double f1 = coeff1 == 0.0 ? 1.0 : pow(inv, coeff1);
@@ -360,7 +377,13 @@ __device__ __forceinline__ void integrate_mode(const Params<NS> &P, int ci, int 
     tprev = nt0;
     tnext = nt1;
     ++steps;
+    if (keep ? any_nonfinite<NS>(y) : !(nt1 > nt0)) {
+      // a non-finite accepted state, or a step size shrunk to nothing
+      status = DEB_STATUS_NONFINITE;
+      break;
+    }
   }
+  if (status != DEB_STATUS_OK) fill_unreached(ydst, iout, P.nout, width);
   if constexpr (TAPE) {
     tw.finish();
     if (status == DEB_STATUS_OK && tw.full) status = DEB_STATUS_TAPE_FULL;
@@ -444,17 +467,13 @@ __device__ __forceinline__ void integrate_batch(const Params<NS> &P, int ci, int
       if (lane == L_ETA) ew = k * k;
       if (lane == L_TB) ew = 1.0 / (k * k);
       const double sc = err[0] / (cf.atol + fmax(fabs(y[0]), fabs(y1[0])) * cf.rtol);
-      const double x = sc * ew;
+      const double x = ew != 0.0 ? sc * ew : 0.0;
       e2 += wave_sum1(x * x);
       double *dst = state(b, 1);
 #pragma unroll
       for (int s = 0; s < NS; ++s) dst[s * WAVE] = y1[s];
     }
-    const double nrm = sqrt(e2 / (6.0 * B));
-    if (!isfinite(nrm)) {
-      status = DEB_STATUS_NONFINITE;
-      break;
-    }
+    const double nrm = norm_or_inf(sqrt(e2 / (6.0 * B)));
     const bool keep = nrm < 1.0;
     double inv = 1.0 / nrm;
     double f1 = coeff1 == 0.0 ? 1.0 : pow(inv, coeff1);
@@ -479,12 +498,19 @@ __device__ __forceinline__ void integrate_batch(const Params<NS> &P, int ci, int
       }
       prev_prev_inv = prev_inv;
       prev_inv = inv;
+      bool bad = false;
       for (int b = 0; b < B; ++b) {
         const double *src = state(b, 1);
         double *dst = state(b, 0);
+        double y[NS];
 #pragma unroll
-        for (int s = 0; s < NS; ++s) dst[s * WAVE] = src[s * WAVE];
+        for (int s = 0; s < NS; ++s) {
+          y[s] = src[s * WAVE];
+          dst[s * WAVE] = y[s];
+        }
+        bad = bad || any_nonfinite<NS>(y);
       }
+      if (bad) status = DEB_STATUS_NONFINITE;
     }
     double nt0 = keep ? tnext : tprev;
     double nt1 = nt0 + dtn;
@@ -492,7 +518,12 @@ __device__ __forceinline__ void integrate_batch(const Params<NS> &P, int ci, int
     tprev = nt0;
     tnext = nt1;
     ++steps;
+    if (!keep && !(nt1 > nt0)) status = DEB_STATUS_NONFINITE;
+    if (status != DEB_STATUS_OK) break;
   }
+  if (status != DEB_STATUS_OK)
+    for (int b = 0; b < B; ++b)
+      fill_unreached(P.y_out + (mode0 + b) * P.nout * (int64_t)width, iout, P.nout, width);
   if (lane == 0) {
     for (int b = 0; b < B; ++b) {
       P.status[mode0 + b] = status;

@@ -160,9 +160,10 @@ inline int hip_check(hipError_t e, const char *what) {
   return 0;
 }
 
-inline int tape_view(TapeView &tv, void *tape, size_t bytes, long long n_modes, int ns) {
+inline int tape_view(TapeView &tv, void *tape, size_t bytes, long long n_modes, int ns, int ckpt) {
+  if (ckpt < 1 || ckpt > TAPE_MAX_CKPT) return fail(DEB_E_ARG, "checkpoint interval outside [1, 4096]");
   const long long pool = tape_pool_off(n_modes);
-  const long long cb = tape_chunk_bytes(ns);
+  const long long cb = tape_chunk_bytes(ns, ckpt);
   if (!tape || (long long)bytes < pool + cb) return fail(DEB_E_ARG, "tape smaller than deb_tape_bytes");
   char *base = static_cast<char *>(tape);
   tv.counter = reinterpret_cast<unsigned long long *>(base);
@@ -171,6 +172,8 @@ inline int tape_view(TapeView &tv, void *tape, size_t bytes, long long n_modes, 
   tv.count = reinterpret_cast<int32_t *>(base + tape_count_off(n_modes));
   tv.pool = base + pool;
   tv.chunk_bytes = cb;
+  tv.state_off = tape_state_off(ckpt);
+  tv.ckpt = ckpt;
   return 0;
 }
 

@@ -121,6 +121,21 @@ __global__ void __launch_bounds__(WAVE) deb_step_kernel(const Params<NS> P, cons
     }
 }
 
+// test hook: the start time of every (cosmology, k) mode
+// (determine_starting_time and 0.99 min(min tau_out, tau*))
+__global__ void __launch_bounds__(WAVE) deb_start_kernel(const Params<1> P, double *ts) {
+  const int item = blockIdx.x;
+  if (item >= P.n_cosmo * P.nk) return;
+  const int ci = item / P.nk;
+  Ctx cx;
+  make_ctx<1>(cx, P, ci, P.kmodes[item]);
+  const double *tout = P.tau_out + (int64_t)ci * P.nout;
+  double tmin = tout[0];
+  for (int i = 1; i < P.nout; ++i) tmin = fmin(tmin, tout[i]);
+  const double t = start_time(cx, tmin);
+  if (lane_id() == 0) ts[item] = t;
+}
+
 template <int NS>
 int launch_forward(const Params<NS> &P, hipStream_t st) {
   const int64_t total = (int64_t)P.n_cosmo * P.nk;
@@ -160,11 +175,17 @@ int run_forward_batched(const deb_config_t *cfg, const Layout &L, int n_cosmo, c
   return hip_check(hipGetLastError(), "batched forward kernel launch");
 }
 
+// clears the tape's chunk counter (a kernel rather than a memset, so that a
+// caller may capture the launch sequence in a graph)
+__global__ void deb_tape_reset_kernel(unsigned long long *counter) {
+  if (threadIdx.x == 0) *counter = 0ull;
+}
+
 template <int NS>
 int run_forward_tape(const deb_config_t *cfg, const Layout &L, int n_cosmo, const deb_cosmo_t *cosmo,
                      const double *tables, int64_t stride, const int32_t *desc, const double *q, const double *w,
                      int nk, const double *kmodes, int nout, const double *tau_out, double *y_out, int32_t *status,
-                     int32_t *nsteps, void *tape, size_t tape_bytes, hipStream_t st) {
+                     int32_t *nsteps, int ckpt, void *tape, size_t tape_bytes, hipStream_t st) {
   Params<NS> P;
   if (int r = fill_params<NS>(P, cfg, L, n_cosmo, cosmo, tables, stride, desc, q, w, nk, kmodes, nout, tau_out))
     return r;
@@ -172,8 +193,9 @@ int run_forward_tape(const deb_config_t *cfg, const Layout &L, int n_cosmo, cons
   P.status = status;
   P.nsteps = nsteps;
   TapeView tv;
-  if (int r = tape_view(tv, tape, tape_bytes, (long long)n_cosmo * nk, NS)) return r;
-  if (int r = hip_check(hipMemsetAsync(tape, 0, 256, st), "tape header reset")) return r;
+  if (int r = tape_view(tv, tape, tape_bytes, (long long)n_cosmo * nk, NS, ckpt)) return r;
+  hipLaunchKernelGGL(deb_tape_reset_kernel, dim3(1), dim3(WAVE), 0, st, tv.counter);
+  if (int r = hip_check(hipGetLastError(), "tape reset launch")) return r;
   hipLaunchKernelGGL(deb_forward_tape_kernel<NS>, dim3((unsigned)((int64_t)n_cosmo * nk)), dim3(WAVE), 0, st, P, tv);
   return hip_check(hipGetLastError(), "taped forward kernel launch");
 }
@@ -264,7 +286,7 @@ int deb_solve_forward_batched(const deb_config_t *cfg, int n_cosmo, const deb_co
 int deb_solve_forward_tape(const deb_config_t *cfg, int n_cosmo, const deb_cosmo_t *cosmo, const double *tables,
                            int64_t table_stride, const int32_t *desc, const double *q, const double *w, int nk,
                            const double *kmodes, int nout, const double *tau_out, double *y_out, int32_t *status,
-                           int32_t *nsteps, void *tape, size_t tape_bytes, void *stream) {
+                           int32_t *nsteps, int ckpt_every, void *tape, size_t tape_bytes, void *stream) {
   if (int r = check_common(cfg, n_cosmo, cosmo, tables, desc, q, w)) return r;
   if (nk < 1 || nout < 1 || !kmodes || !tau_out || !y_out || !status) return fail(DEB_E_ARG, "invalid mode arrays");
   if ((int64_t)n_cosmo * nk > 0x7fffffff) return fail(DEB_E_ARG, "too many modes");
@@ -272,7 +294,23 @@ int deb_solve_forward_tape(const deb_config_t *cfg, int n_cosmo, const deb_cosmo
   if (int r = build_layout(*cfg, L)) return r;
   hipStream_t st = (hipStream_t)stream;
   DEB_DISPATCH(L.ns, (run_forward_tape<NS>(cfg, L, n_cosmo, cosmo, tables, table_stride, desc, q, w, nk, kmodes, nout,
-                                           tau_out, y_out, status, nsteps, tape, tape_bytes, st)))
+                                           tau_out, y_out, status, nsteps, ckpt_every, tape, tape_bytes, st)))
+}
+
+int deb_start_times(const deb_config_t *cfg, int n_cosmo, const deb_cosmo_t *cosmo, const double *tables,
+                    int64_t table_stride, const int32_t *desc, const double *q, const double *w, int nk,
+                    const double *kmodes, int nout, const double *tau_out, double *tau_start, void *stream) {
+  if (int r = check_common(cfg, n_cosmo, cosmo, tables, desc, q, w)) return r;
+  if (nk < 1 || nout < 1 || !kmodes || !tau_out || !tau_start) return fail(DEB_E_ARG, "invalid mode arrays");
+  if ((int64_t)n_cosmo * nk > 0x7fffffff) return fail(DEB_E_ARG, "too many modes");
+  Layout L;
+  if (int r = build_layout(*cfg, L)) return r;
+  Params<1> P;
+  if (int r = fill_params<1>(P, cfg, L, n_cosmo, cosmo, tables, table_stride, desc, q, w, nk, kmodes, nout, tau_out))
+    return r;
+  hipLaunchKernelGGL(deb_start_kernel, dim3((unsigned)((int64_t)n_cosmo * nk)), dim3(WAVE), 0, (hipStream_t)stream,
+                     P, tau_start);
+  return hip_check(hipGetLastError(), "start-time kernel launch");
 }
 
 int deb_rhs(const deb_config_t *cfg, const deb_cosmo_t *cosmo, const double *tables, int64_t table_stride,

@@ -39,6 +39,8 @@ def lib():
         L.deb_solve_forward.argtypes = [ctypes.POINTER(DebConfig), ctypes.c_int, _dp, _dp, _i64, _dp, _dp, _dp,
                                         ctypes.c_int, _dp, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, ctypes.c_size_t,
                                         _dp]
+        L.deb_start_times.argtypes = [ctypes.POINTER(DebConfig), ctypes.c_int, _dp, _dp, _i64, _dp, _dp, _dp,
+                                      ctypes.c_int, _dp, ctypes.c_int, _dp, _dp, _dp]
         L.deb_rhs.argtypes = [ctypes.POINTER(DebConfig), _dp, _dp, _i64, _dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp,
                               _dp, _dp, _dp]
         L.deb_rodas5_step.argtypes = [ctypes.POINTER(DebConfig), _dp, _dp, _i64, _dp, _dp, _dp, ctypes.c_int, _dp,
@@ -48,20 +50,22 @@ def lib():
         L.deb_solve_forward_batched.argtypes = [ctypes.POINTER(DebConfig), ctypes.c_int, _dp, _dp, _i64, _dp, _dp,
                                                 _dp, ctypes.c_int, _dp, ctypes.c_int, _dp, ctypes.c_int, _dp, _dp,
                                                 _dp, _dp, ctypes.c_size_t, _dp]
-        L.deb_tape_bytes.argtypes = [ctypes.POINTER(DebConfig), ctypes.c_int, ctypes.c_int, _i64,
+        L.deb_tape_bytes.argtypes = [ctypes.POINTER(DebConfig), ctypes.c_int, ctypes.c_int, _i64, ctypes.c_int,
                                      ctypes.POINTER(ctypes.c_size_t)]
         L.deb_solve_forward_tape.argtypes = [ctypes.POINTER(DebConfig), ctypes.c_int, _dp, _dp, _i64, _dp, _dp, _dp,
-                                             ctypes.c_int, _dp, ctypes.c_int, _dp, _dp, _dp, _dp, _dp,
+                                             ctypes.c_int, _dp, ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_int, _dp,
                                              ctypes.c_size_t, _dp]
+        L.deb_adjoint_workspace_bytes.argtypes = [ctypes.POINTER(DebConfig), ctypes.c_int,
+                                                  ctypes.POINTER(ctypes.c_size_t)]
         L.deb_solve_adjoint.argtypes = [ctypes.POINTER(DebConfig), ctypes.c_int, _dp, _dp, _i64, _dp, _dp, _dp,
                                         ctypes.c_int, _dp, ctypes.c_int, _dp, ctypes.c_int, _dp, _dp, _dp, _dp, _dp,
-                                        _dp, _dp, ctypes.c_size_t, _dp]
+                                        _dp, ctypes.c_int, _dp, ctypes.c_size_t, _dp, ctypes.c_size_t, _dp]
         L.deb_rhs_vjp.argtypes = [ctypes.POINTER(DebConfig), _dp, _dp, _i64, _dp, _dp, _dp, ctypes.c_int, _dp, _dp,
                                   ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
         L.deb_rodas5_step_vjp.argtypes = [ctypes.POINTER(DebConfig), _dp, _dp, _i64, _dp, _dp, _dp, ctypes.c_int,
                                           _dp, _dp, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
                                           _dp]
-        if L.deb_abi_version() != 1:
+        if L.deb_abi_version() != 2:
             raise DebError("libdiscoeb_hip.so ABI version mismatch")
         _LIB = L
     return _LIB

@@ -62,6 +62,14 @@ def sample_thetas(n: int, seed: int = SEED) -> np.ndarray:
     return rng.uniform(THETA_LO, THETA_HI, size=(n, 8))
 
 
+def loss_weights(n_k: int, n_cosmo: int, seed: int = SEED) -> np.ndarray:
+    """w(k) ~ U[0, 1] of the batched-adjoint loss sum_k w(k) ln P(k): drawn from
+    the same generator as the cosmologies, after the n_cosmo draws."""
+    rng = np.random.default_rng(seed)
+    rng.uniform(THETA_LO, THETA_HI, size=(n_cosmo, 8))
+    return rng.uniform(0.0, 1.0, size=n_k)
+
+
 def test_cosmology_param() -> dict:
     """The reference test suite's cosmology (tests/test_perturbations.py:12-57)."""
     Tnu = (4 / 11) ** (1 / 3)

@@ -25,24 +25,41 @@ from .perturbations import DeviceInputs, _torch
 FIELD_DELTA_M = 4
 
 
+CKPT_EVERY = 64            # accepted steps per checkpoint segment
+TAPE_BUDGET = 64 * 2**30   # default device bytes for the tape and the reverse launch's rings
+STATES_PER_MODE = 4096     # first estimate of the accepted states of a mode
+
+
 class Tape:
     """Device checkpoint tape of one taped forward solve."""
 
-    def __init__(self, buf, nbytes):
+    def __init__(self, buf, nbytes, ckpt_every):
         self.buf = buf
         self.nbytes = int(nbytes)
+        self.ckpt_every = int(ckpt_every)
 
 
-def tape_bytes(cfg, n_cosmo: int, nk: int, n_states: int) -> int:
+def tape_bytes(cfg, n_cosmo: int, nk: int, n_states: int, ckpt_every: int = CKPT_EVERY) -> int:
     b = ctypes.c_size_t(0)
-    _lib.check(_lib.lib().deb_tape_bytes(ctypes.byref(cfg), n_cosmo, nk, int(n_states), ctypes.byref(b)),
-               "deb_tape_bytes")
+    _lib.check(_lib.lib().deb_tape_bytes(ctypes.byref(cfg), n_cosmo, nk, int(n_states), int(ckpt_every),
+                                         ctypes.byref(b)), "deb_tape_bytes")
     return int(b.value)
 
 
-def solve_forward_tape(inp: DeviceInputs, kmodes, tau_out, *, n_states: int | None = None, stream=None):
-    """Taped forward solve.  ``n_states``: tape size in states (default: a
-    generous estimate; on overflow the solve is repeated with the exact count)."""
+def adjoint_workspace_bytes(cfg, ckpt_every: int = CKPT_EVERY) -> int:
+    b = ctypes.c_size_t(0)
+    _lib.check(_lib.lib().deb_adjoint_workspace_bytes(ctypes.byref(cfg), int(ckpt_every), ctypes.byref(b)),
+               "deb_adjoint_workspace_bytes")
+    return int(b.value)
+
+
+def solve_forward_tape(inp: DeviceInputs, kmodes, tau_out, *, n_states: int | None = None,
+                       ckpt_every: int = CKPT_EVERY, budget_bytes: int = TAPE_BUDGET, stream=None):
+    """Taped forward solve: every mode's accepted states in segments of
+    ``ckpt_every`` steps (their first state and every step time are kept).
+    ``n_states``: tape size in states (default STATES_PER_MODE per mode; on
+    overflow the solve is repeated with the exact count).  The tape plus the
+    reverse launch's workspace must fit ``budget_bytes``: DebError otherwise."""
     torch = _torch()
     L = _lib.lib()
     cfg = inp.cfg
@@ -52,9 +69,14 @@ def solve_forward_tape(inp: DeviceInputs, kmodes, tau_out, *, n_states: int | No
     nout = tau_out.shape[1]
     st = stream if stream is not None else torch.cuda.current_stream(inp.device)
     if n_states is None:
-        n_states = n_cosmo * nk * 2048
+        n_states = n_cosmo * nk * STATES_PER_MODE
+    ws = adjoint_workspace_bytes(cfg, ckpt_every)
     for attempt in range(2):
-        nb = tape_bytes(cfg, n_cosmo, nk, n_states)
+        nb = tape_bytes(cfg, n_cosmo, nk, n_states, ckpt_every)
+        if nb + ws > budget_bytes:
+            raise _lib.DebError(f"checkpoint tape ({nb / 2**30:.1f} GiB for {n_states} states) and reverse-launch "
+                                f"workspace ({ws / 2**30:.1f} GiB) exceed the budget of {budget_bytes / 2**30:.1f} "
+                                f"GiB; raise ckpt_every ({ckpt_every}) or the budget")
         buf = torch.empty(nb, dtype=torch.uint8, device=inp.device)
         y = torch.empty((n_cosmo, nk, nout, packing.N_OUT_FIELDS), dtype=torch.float64, device=inp.device)
         status = torch.full((n_cosmo, nk), -1, dtype=torch.int32, device=inp.device)
@@ -62,11 +84,11 @@ def solve_forward_tape(inp: DeviceInputs, kmodes, tau_out, *, n_states: int | No
         _lib.check(L.deb_solve_forward_tape(ctypes.byref(cfg), n_cosmo, _lib.ptr(inp.cosmo), _lib.ptr(inp.tables),
                                             inp.stride, _lib.ptr(inp.desc), _lib.ptr(inp.q), _lib.ptr(inp.w), nk,
                                             _lib.ptr(kmodes), nout, _lib.ptr(tau_out), _lib.ptr(y),
-                                            _lib.ptr(status), _lib.ptr(nsteps), _lib.ptr(buf), nb,
+                                            _lib.ptr(status), _lib.ptr(nsteps), int(ckpt_every), _lib.ptr(buf), nb,
                                             ctypes.c_void_p(st.cuda_stream)), "deb_solve_forward_tape")
         full = bool((status == packing.STATUS_TAPE_FULL).any())
         if not full:
-            return y, status, nsteps, Tape(buf, nb)
+            return y, status, nsteps, Tape(buf, nb, ckpt_every)
         # nsteps counts attempted steps: an upper bound of the accepted states
         n_states = int(nsteps.sum().item()) + n_cosmo * nk
         del buf
@@ -95,13 +117,15 @@ def solve_adjoint(inp: DeviceInputs, tan: Tangents, kmodes, tau_out, ybar, statu
     n_cosmo, nk = kmodes.shape
     nout = tau_out.shape[1]
     grad = torch.empty((n_cosmo, nk, tan.ndir), dtype=torch.float64, device=inp.device)
+    wsb = adjoint_workspace_bytes(cfg, tape.ckpt_every)
+    ws = torch.empty(wsb, dtype=torch.uint8, device=inp.device)
     st = stream if stream is not None else torch.cuda.current_stream(inp.device)
     _lib.check(L.deb_solve_adjoint(ctypes.byref(cfg), n_cosmo, _lib.ptr(inp.cosmo), _lib.ptr(inp.tables), inp.stride,
                                    _lib.ptr(inp.desc), _lib.ptr(inp.q), _lib.ptr(inp.w), nk, _lib.ptr(kmodes), nout,
                                    _lib.ptr(tau_out), tan.ndir, _lib.ptr(tan.dcos), _lib.ptr(tan.dtab),
                                    _lib.ptr(tan.dtau), _lib.ptr(ybar), _lib.ptr(status), _lib.ptr(grad),
-                                   _lib.ptr(tape.buf), tape.nbytes, ctypes.c_void_p(st.cuda_stream)),
-               "deb_solve_adjoint")
+                                   tape.ckpt_every, _lib.ptr(tape.buf), tape.nbytes, _lib.ptr(ws), wsb,
+                                   ctypes.c_void_p(st.cuda_stream)), "deb_solve_adjoint")
     return grad
 
 

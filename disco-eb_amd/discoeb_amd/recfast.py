@@ -1,5 +1,26 @@
 """RECFAST-style recombination history (host-side table producer).
 
+This file is derived from RECFAST through DISCO-EB's
+thermodynamics_recfast.py and carries the RECFAST copyright notice, which its
+licence requires on all copies and modifications:
+
+ # Integrator for Cosmic Recombination of Hydrogen and Helium,
+ # developed by Douglas Scott (dscott@astro.ubc.ca)
+ # based on calculations in the papers Seager, Sasselov & Scott
+ # (ApJ, 523, L1, 1999; ApJS, 128, 407, 2000)
+ # and "fudge" updates in Wong, Moss & Scott (2008).
+ #
+ # Permission to use, copy, modify and distribute without fee or royalty at
+ # any tier, this software and its documentation, for any purpose and without
+ # fee or royalty is hereby granted, provided that you agree to comply with
+ # the following copyright notice and statements, including the disclaimer,
+ # and that the same appear on ALL copies of the software and documentation,
+ # including modifications that you make for internal use or for distribution:
+ #
+ # Copyright 1999-2010 by University of British Columbia.  All rights reserved.
+
+The original RECFAST code: https://www.astro.ubc.ca/people/scott/recfast.html
+
 Follows ``src/discoeb/thermodynamics_recfast.py`` of DISCO-EB, itself adapted
 from RECFAST (Seager, Sasselov & Scott 1999/2000; Wong, Moss & Scott 2008):
 

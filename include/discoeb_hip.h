@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define DEB_ABI_VERSION 1
+#define DEB_ABI_VERSION 2
 #define DEB_MAXNQ 16    /* maximum massive-neutrino momentum bins */
 #define DEB_NOUT_FIELDS 20
 #define DEB_MAXDIR 6    /* input tangent directions per gradient launch */
@@ -49,11 +49,15 @@ enum {
   DEB_NSPLINE = 7
 };
 
-/* Per-mode status (the reference raises for the whole call; see throw=). */
+/* Per-mode status (the reference raises for the whole call; see throw=).
+ * A NaN or inf error norm rejects the step (diffrax counts a NaN error as inf)
+ * and shrinks it by factormin.  A mode that fails leaves inf in every output
+ * it did not reach (diffrax's unreached SaveAt entries). */
 enum {
   DEB_STATUS_OK = 0,
   DEB_STATUS_MAX_STEPS = 1, /* diffrax max_steps reached before t1 */
-  DEB_STATUS_NONFINITE = 2, /* NaN/inf error norm                   */
+  DEB_STATUS_NONFINITE = 2, /* a non-finite accepted state, or the step size
+                               shrunk to nothing after rejections */
   DEB_STATUS_TAPE_FULL = 3, /* checkpoint tape exhausted (gradient path); the
                                forward results are complete, the tape is not */
   DEB_STATUS_NOT_RUN = -1
@@ -151,6 +155,16 @@ int deb_solve_forward_batched(const deb_config_t *cfg, int n_cosmo, const deb_co
                               void *stream);
 
 /*
+ * Test hook: the start time of every (cosmology, k) mode, as deb_solve_forward
+ * computes it: determine_starting_time (perturbations.py:630-681, bisection
+ * util.py:365-396) and tau_start = 0.99 min(min tau_out, tau*) (:743).
+ * Arguments as deb_solve_forward; tau_start device [n_cosmo][nk].
+ */
+int deb_start_times(const deb_config_t *cfg, int n_cosmo, const deb_cosmo_t *cosmo, const double *tables,
+                    int64_t table_stride, const int32_t *desc, const double *q, const double *w, int nk,
+                    const double *kmodes, int nout, const double *tau_out, double *tau_start, void *stream);
+
+/*
  * Operator-level test hook: f = model_synchronous(tau, y) (perturbations.py:84-371)
  * for m independent (cosmology index, k, tau, y) tuples, evaluated by the
  * kernel's wave-parallel right-hand side.
@@ -189,33 +203,53 @@ int deb_rodas5_step(const deb_config_t *cfg, const deb_cosmo_t *cosmo, const dou
  * ybar . d(y_out)/d(direction j).
  */
 
-/* Bytes of a checkpoint tape for n_states accepted states in total (sum over
- * modes of accepted steps + 1).  A sufficient bound: sum of nsteps + n_modes. */
-int deb_tape_bytes(const deb_config_t *cfg, int n_cosmo, int nk, int64_t n_states, size_t *bytes);
+/*
+ * Checkpoint tape: the accepted states of every mode are grouped in segments
+ * of ckpt_every consecutive steps; the tape keeps each segment's first state
+ * and every step's time (one 512 NS-byte state per segment plus 8 bytes per
+ * step, NS = deb_slots), and the reverse launch
+ * recomputes each segment forward from its checkpoint.  ckpt_every = 1 keeps
+ * every state.
+ *
+ * Bytes of a tape for n_states accepted states in total (sum over modes of
+ * accepted steps + 1).  A sufficient bound: sum of nsteps + n_modes.
+ */
+int deb_tape_bytes(const deb_config_t *cfg, int n_cosmo, int nk, int64_t n_states, int ckpt_every, size_t *bytes);
 
 /*
- * deb_solve_forward that also records every accepted state of every mode on
- * the tape (device memory of tape_bytes, owned by the caller).  Modes whose
- * states do not fit get DEB_STATUS_TAPE_FULL (their outputs are complete).
- * Arguments as deb_solve_forward; return_full must be 0 for a later
- * deb_solve_adjoint.
+ * deb_solve_forward that also records every mode on the checkpoint tape
+ * (device memory of tape_bytes, owned by the caller).  Modes whose segments do
+ * not fit get DEB_STATUS_TAPE_FULL (their outputs are complete).  Arguments as
+ * deb_solve_forward; ckpt_every in [1, 4096]; return_full must be 0 for a
+ * later deb_solve_adjoint.
  */
 int deb_solve_forward_tape(const deb_config_t *cfg, int n_cosmo, const deb_cosmo_t *cosmo, const double *tables,
                            int64_t table_stride, const int32_t *desc, const double *q, const double *w, int nk,
                            const double *kmodes, int nout, const double *tau_out, double *y_out, int32_t *status,
-                           int32_t *nsteps, void *tape, size_t tape_bytes, void *stream);
+                           int32_t *nsteps, int ckpt_every, void *tape, size_t tape_bytes, void *stream);
+
+/*
+ * Bytes of device workspace deb_solve_adjoint needs: one ring per wave the
+ * current device can hold at the reverse kernel's occupancy, times two (each
+ * ring holds a segment's ckpt_every + 1 states and 8 ckpt_every stage vectors,
+ * 512 NS (9 ckpt_every + 1) bytes).  Queries the current HIP device.
+ */
+int deb_adjoint_workspace_bytes(const deb_config_t *cfg, int ckpt_every, size_t *bytes);
 
 /*
  * Reverse sweep over a tape written by deb_solve_forward_tape with the same
- * inputs.  ybar device [n_cosmo][nk][nout][20] cotangents of y_out; status
- * device [n_cosmo][nk] from the taped solve (modes not OK get NaN); grad
- * device [n_cosmo][nk][ndir]; dcosmo, dtables, dtau_out device (see above).
+ * inputs and ckpt_every.  ybar device [n_cosmo][nk][nout][20] cotangents of
+ * y_out; status device [n_cosmo][nk] from the taped solve (modes not OK get
+ * NaN); grad device [n_cosmo][nk][ndir]; dcosmo, dtables, dtau_out device
+ * (see above); workspace device >= deb_adjoint_workspace_bytes (its contents
+ * need no initialisation).
  */
 int deb_solve_adjoint(const deb_config_t *cfg, int n_cosmo, const deb_cosmo_t *cosmo, const double *tables,
                       int64_t table_stride, const int32_t *desc, const double *q, const double *w, int nk,
                       const double *kmodes, int nout, const double *tau_out, int ndir, const double *dcosmo,
                       const double *dtables, const double *dtau_out, const double *ybar, const int32_t *status,
-                      double *grad, const void *tape, size_t tape_bytes, void *stream);
+                      double *grad, int ckpt_every, const void *tape, size_t tape_bytes, void *workspace,
+                      size_t workspace_bytes, void *stream);
 
 /*
  * Operator-level test hook: reverse of model_synchronous (perturbations.py:84-371)

@@ -37,6 +37,7 @@
 // directional derivative of the discrete solution.
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -651,11 +652,60 @@ void rodas5_step(const ModelT<P> &m, const T &t0, const T &t1, const T *y0, T *y
   rodas5_step_sys<T>(ModeSystem<P>{m}, t0, t1, y0, y1, err, w);
 }
 
+inline bool all_finite(const std::vector<double> &y) {
+  for (double v : y)
+    if (!std::isfinite(v)) return false;
+  return true;
+}
+
 struct SolverCfg {
   double rtol, atol, pcoeff, icoeff, dcoeff, factormax, factormin;
   int max_steps;
   int return_full;
 };
+
+// Sum of squares of the filtered, weighted scaled error: rms_norm_filtered
+// (perturbations.py:701-705, indices a, eta, delta_c, delta_b, theta_b,
+// delta_gamma with weights 1, k^2, 1, 1, 1/k^2, 1) of diffrax's scaled error
+// err / (atol + rtol max(|y0|, |y1|)).  Entries outside the filter are never read.
+double filtered_sq(const double *err, const double *y0, const double *y1, double k, const SolverCfg &s) {
+  const int fidx[6] = {0, 2, 3, 5, 6, 7};
+  const double fw[6] = {1.0, k * k, 1.0, 1.0, 1.0 / (k * k), 1.0};
+  double ss = 0.0;
+  for (int j = 0; j < 6; ++j) {
+    const int i = fidx[j];
+    const double sc = err[i] / (s.atol + std::max(std::fabs(y0[i]), std::fabs(y1[i])) * s.rtol);
+    const double x = sc * fw[j];
+    ss += x * x;
+  }
+  return ss;
+}
+
+// diffrax PIDController.adapt_step_size for one attempted step: accept if the
+// norm is < 1; a NaN error counts as inf (rejected, step shrunk by factormin);
+// inv is the inverse norm carried to the next steps (reset to 1 when 0 or inf)
+struct Control {
+  bool keep;
+  double factor, inv;
+};
+Control pid_control(double nrm, double prev_inv, double prev_prev_inv, const SolverCfg &s) {
+  const double eo = 5.0;
+  const double coeff1 = (s.icoeff + s.pcoeff + s.dcoeff) / eo;
+  const double coeff2 = -(s.pcoeff + 2.0 * s.dcoeff) / eo;
+  const double coeff3 = s.dcoeff / eo;
+  if (!std::isfinite(nrm)) nrm = INFINITY;
+  Control c;
+  c.keep = nrm < 1.0;
+  double inv = 1.0 / nrm;
+  const double f1 = coeff1 == 0.0 ? 1.0 : std::pow(inv, coeff1);
+  const double f2 = coeff2 == 0.0 ? 1.0 : std::pow(prev_inv, coeff2);
+  const double f3 = coeff3 == 0.0 ? 1.0 : std::pow(prev_prev_inv, coeff3);
+  const double fmin = c.keep ? 1.0 : s.factormin;
+  c.factor = std::min(std::max(0.9 * f1 * f2 * f3, fmin), s.factormax);
+  if (inv == 0.0 || std::isinf(inv)) inv = 1.0;
+  c.inv = inv;
+  return c;
+}
 
 // evolve_one_mode: returns status; fills out[nout * (return_full ? n : 20)].
 // times (optional) receives t0 and the end time of every accepted step.
@@ -676,36 +726,17 @@ int evolve_mode(const Model &m, const SolverCfg &s, const double *tau_out, int n
   const double dt0 = std::min(t0 / 4.0, 0.5 * (t1 - t0));
   double tprev = t0, tnext = std::min(t0 + dt0, t1);
   double prev_inv = 1.0, prev_prev_inv = 1.0;
-  const double eo = 5.0;
-  const double coeff1 = (s.icoeff + s.pcoeff + s.dcoeff) / eo;
-  const double coeff2 = -(s.pcoeff + 2.0 * s.dcoeff) / eo;
-  const double coeff3 = s.dcoeff / eo;
-  const int fidx[6] = {0, 2, 3, 5, 6, 7};
-  const double fw[6] = {1.0, m.k * m.k, 1.0, 1.0, 1.0 / (m.k * m.k), 1.0};
   int iout = 0, steps = 0, acc = 0;
   const int width = s.return_full ? n : 20;
   int status = DEB_STATUS_OK;
   while (tprev < t1) {
     if (steps >= s.max_steps) { status = DEB_STATUS_MAX_STEPS; break; }
     rodas5_step<double, double>(m, tprev, tnext, y.data(), y1.data(), err.data(), w);
-    double ss = 0.0;
-    for (int j = 0; j < 6; ++j) {
-      int i = fidx[j];
-      double sc = err[i] / (s.atol + std::max(std::fabs(y[i]), std::fabs(y1[i])) * s.rtol);
-      double x = sc * fw[j];
-      ss += x * x;
-    }
-    double nrm = std::sqrt(ss / 6.0);
-    if (!std::isfinite(nrm)) { status = DEB_STATUS_NONFINITE; break; }
-    bool keep = nrm < 1.0;
-    double inv = 1.0 / nrm;
-    double f1 = coeff1 == 0.0 ? 1.0 : std::pow(inv, coeff1);
-    double f2 = coeff2 == 0.0 ? 1.0 : std::pow(prev_inv, coeff2);
-    double f3 = coeff3 == 0.0 ? 1.0 : std::pow(prev_prev_inv, coeff3);
-    double fmin = keep ? 1.0 : s.factormin;
-    double factor = std::min(std::max(0.9 * f1 * f2 * f3, fmin), s.factormax);
-    double dt = (tnext - tprev) * factor;
-    if (inv == 0.0 || std::isinf(inv)) inv = 1.0;
+    const double nrm = std::sqrt(filtered_sq(err.data(), y.data(), y1.data(), m.k, s) / 6.0);
+    const Control ctl = pid_control(nrm, prev_inv, prev_prev_inv, s);
+    const bool keep = ctl.keep;
+    const double inv = ctl.inv;
+    double dt = (tnext - tprev) * ctl.factor;
     if (keep) {
       while (iout < nout && tau_out[iout] <= tnext) {
         double th = (tau_out[iout] - tprev) / (tnext - tprev);
@@ -726,7 +757,11 @@ int evolve_mode(const Model &m, const SolverCfg &s, const double *tau_out, int n
     tprev = nt0;
     tnext = nt1;
     ++steps;
+    // a non-finite accepted state, or a step size shrunk to nothing
+    if (keep ? !all_finite(y) : !(nt1 > nt0)) { status = DEB_STATUS_NONFINITE; break; }
   }
+  // outputs a failed mode never reached hold inf (diffrax's unreached SaveAt entries)
+  if (status != DEB_STATUS_OK) std::fill(out + (size_t)iout * width, out + (size_t)nout * width, INFINITY);
   *nsteps = steps;
   *naccept = acc;
   return status;
@@ -739,7 +774,7 @@ int evolve_mode(const Model &m, const SolverCfg &s, const double *tau_out, int n
 // over all modes' filtered, weighted errors (rms_norm_filtered_batched
 // :690-694).  out[b][nout][width]; every mode gets the group's status.
 int evolve_batch(const std::vector<Model> &ms, const SolverCfg &s, const double *tau_out, int nout, double *out,
-                 int *nsteps, int *naccept) {
+                 int *nsteps, int *naccept, std::vector<double> *trace = nullptr) {
   const int B = (int)ms.size();
   const int n = ms[0].nvar();
   Work<double> w(n);
@@ -757,38 +792,23 @@ int evolve_batch(const std::vector<Model> &ms, const SolverCfg &s, const double 
   const double dt0 = std::min(t0 / 4.0, 0.5 * (t1 - t0));
   double tprev = t0, tnext = std::min(t0 + dt0, t1);
   double prev_inv = 1.0, prev_prev_inv = 1.0;
-  const double eo = 5.0;
-  const double coeff1 = (s.icoeff + s.pcoeff + s.dcoeff) / eo;
-  const double coeff2 = -(s.pcoeff + 2.0 * s.dcoeff) / eo;
-  const double coeff3 = s.dcoeff / eo;
-  const int fidx[6] = {0, 2, 3, 5, 6, 7};
   int iout = 0, steps = 0, acc = 0;
   const int width = s.return_full ? n : 20;
   int status = DEB_STATUS_OK;
   while (tprev < t1) {
     if (steps >= s.max_steps) { status = DEB_STATUS_MAX_STEPS; break; }
+    // rms_norm_filtered_batched (:690-694): one RMS over every mode's 6 entries
     double ss = 0.0;
     for (int b = 0; b < B; ++b) {
       rodas5_step<double, double>(ms[b], tprev, tnext, y[b].data(), y1[b].data(), err.data(), w);
-      const double fw[6] = {1.0, ms[b].k * ms[b].k, 1.0, 1.0, 1.0 / (ms[b].k * ms[b].k), 1.0};
-      for (int j = 0; j < 6; ++j) {
-        int i = fidx[j];
-        double sc = err[i] / (s.atol + std::max(std::fabs(y[b][i]), std::fabs(y1[b][i])) * s.rtol);
-        double x = sc * fw[j];
-        ss += x * x;
-      }
+      ss += filtered_sq(err.data(), y[b].data(), y1[b].data(), ms[b].k, s);
     }
-    double nrm = std::sqrt(ss / (6.0 * B));
-    if (!std::isfinite(nrm)) { status = DEB_STATUS_NONFINITE; break; }
-    bool keep = nrm < 1.0;
-    double inv = 1.0 / nrm;
-    double f1 = coeff1 == 0.0 ? 1.0 : std::pow(inv, coeff1);
-    double f2 = coeff2 == 0.0 ? 1.0 : std::pow(prev_inv, coeff2);
-    double f3 = coeff3 == 0.0 ? 1.0 : std::pow(prev_prev_inv, coeff3);
-    double fmin = keep ? 1.0 : s.factormin;
-    double factor = std::min(std::max(0.9 * f1 * f2 * f3, fmin), s.factormax);
-    double dt = (tnext - tprev) * factor;
-    if (inv == 0.0 || std::isinf(inv)) inv = 1.0;
+    const double nrm = std::sqrt(ss / (6.0 * B));
+    if (trace) trace->insert(trace->end(), {tprev, tnext, nrm});
+    const Control ctl = pid_control(nrm, prev_inv, prev_prev_inv, s);
+    const bool keep = ctl.keep;
+    const double inv = ctl.inv;
+    double dt = (tnext - tprev) * ctl.factor;
     if (keep) {
       while (iout < nout && tau_out[iout] <= tnext) {
         double th = (tau_out[iout] - tprev) / (tnext - tprev);
@@ -811,7 +831,13 @@ int evolve_batch(const std::vector<Model> &ms, const SolverCfg &s, const double 
     tprev = nt0;
     tnext = nt1;
     ++steps;
+    bool bad = !keep && !(nt1 > nt0);
+    for (int b = 0; keep && b < B; ++b) bad = bad || !all_finite(y[b]);
+    if (bad) { status = DEB_STATUS_NONFINITE; break; }
   }
+  if (status != DEB_STATUS_OK)
+    for (int b = 0; b < B; ++b)
+      std::fill(out + ((size_t)b * nout + iout) * width, out + ((size_t)b * nout + nout) * width, INFINITY);
   *nsteps = steps;
   *naccept = acc;
   return status;
@@ -999,6 +1025,42 @@ int oracle_rodas5_linear(int n, const double *A, const double *g0, const double 
   return 0;
 }
 
+// The step-size controller's view of one attempted step: out[0] = error norm
+// (filtered RMS), out[1] = 1 if accepted, out[2] = step-size factor, out[3] =
+// the inverse norm carried forward.
+int oracle_step_control(const deb_config_t *cfg, double k, const double *err, const double *y0, const double *y1,
+                        double prev_inv, double prev_prev_inv, double *out) {
+  SolverCfg s{cfg->rtol, cfg->atol, cfg->pcoeff, cfg->icoeff, cfg->dcoeff, cfg->factormax, cfg->factormin,
+              cfg->max_steps, cfg->return_full};
+  const double nrm = std::sqrt(filtered_sq(err, y0, y1, k, s) / 6.0);
+  const Control c = pid_control(nrm, prev_inv, prev_prev_inv, s);
+  out[0] = std::isfinite(nrm) ? nrm : INFINITY;
+  out[1] = c.keep ? 1.0 : 0.0;
+  out[2] = c.factor;
+  out[3] = c.inv;
+  return 0;
+}
+
+// The first nk modes as one group of evolve_modes_batched; trace[3 i + 0..2]
+// = (t0, t1, error norm) of attempted step i for i < max_trace; returns the
+// number of attempted steps (negative on error).
+int oracle_batch_trace(const deb_config_t *cfg, const deb_cosmo_t *c, const double *tables, const int32_t *desc,
+                       const double *q, const double *w, int nk, const double *kmodes, int nout,
+                       const double *tau_out, int max_trace, double *trace) {
+  SolverCfg s{cfg->rtol, cfg->atol, cfg->pcoeff, cfg->icoeff, cfg->dcoeff, cfg->factormax, cfg->factormin,
+              cfg->max_steps, cfg->return_full};
+  std::vector<Model> ms;
+  for (int b = 0; b < nk; ++b) ms.push_back(make_model(c, tables, desc, cfg, q, w, kmodes[b]));
+  const int n = oracle_nvar(cfg);
+  std::vector<double> out((size_t)nk * nout * (cfg->return_full ? n : 20));
+  std::vector<double> tr;
+  int ns = 0, na = 0;
+  evolve_batch(ms, s, tau_out, nout, out.data(), &ns, &na, &tr);
+  const int m = std::min<int>(max_trace, (int)tr.size() / 3);
+  std::copy(tr.begin(), tr.begin() + 3 * m, trace);
+  return ns;
+}
+
 // adiabatic initial conditions at an explicit time tau
 int oracle_ics_at(const deb_config_t *cfg, const deb_cosmo_t *c, const double *tables, const int32_t *desc,
                   const double *q, const double *w, double k, double tau, double *y0) {
@@ -1018,7 +1080,7 @@ int oracle_outputs(const deb_config_t *cfg, const deb_cosmo_t *c, const double *
 int oracle_evolve(const deb_config_t *cfg, const deb_cosmo_t *c, const double *tables, const int32_t *desc,
                   const double *q, const double *w, int nk, const double *kmodes, int nout, const double *tau_out,
                   double *y_out, int32_t *status, int32_t *nsteps, int32_t *naccept, double *tau_start,
-                  int nthreads) {
+                  int nthreads, double *seconds) {
   SolverCfg s{cfg->rtol, cfg->atol, cfg->pcoeff, cfg->icoeff, cfg->dcoeff, cfg->factormax, cfg->factormin,
               cfg->max_steps, cfg->return_full};
   const int n = oracle_nvar(cfg);
@@ -1028,10 +1090,12 @@ int oracle_evolve(const deb_config_t *cfg, const deb_cosmo_t *c, const double *t
     for (;;) {
       int i = next.fetch_add(1);
       if (i >= nk) break;
+      const auto clk = std::chrono::steady_clock::now();
       Model m = make_model(c, tables, desc, cfg, q, w, kmodes[i]);
       int st, na;
       double ts;
       status[i] = evolve_mode(m, s, tau_out, nout, y_out + (size_t)i * nout * width, &st, &na, &ts);
+      if (seconds) seconds[i] = std::chrono::duration<double>(std::chrono::steady_clock::now() - clk).count();
       nsteps[i] = st;
       naccept[i] = na;
       if (tau_start) tau_start[i] = ts;
@@ -1167,7 +1231,7 @@ int oracle_evolve_tangent(const deb_config_t *cfg, const deb_cosmo_t *c, const d
                           const deb_cosmo_t *dc, const double *dtables, const int32_t *desc, const double *q,
                           const double *w, int nk, const double *kmodes, int nout, const double *tau_out,
                           const double *dtau_out, double *y_out, double *dy_out, int32_t *status, int32_t *naccept,
-                          int nthreads) {
+                          int nthreads, double *seconds) {
   SolverCfg s{cfg->rtol, cfg->atol, cfg->pcoeff, cfg->icoeff, cfg->dcoeff, cfg->factormax, cfg->factormin,
               cfg->max_steps, 0};
   std::atomic<int> next(0);
@@ -1176,11 +1240,13 @@ int oracle_evolve_tangent(const deb_config_t *cfg, const deb_cosmo_t *c, const d
     for (;;) {
       int i = next.fetch_add(1);
       if (i >= nk) break;
+      const auto clk = std::chrono::steady_clock::now();
       Model m = make_model(c, tables, desc, cfg, q, w, kmodes[i]);
       ModelT<Dd> md = make_model_tangent(c, tables, dc, dtables, desc, cfg, q, w, kmodes[i], store);
       int na = 0;
       status[i] = evolve_tangent(m, md, s, tau_out, dtau_out, nout, y_out + (size_t)i * nout * 20,
                                  dy_out + (size_t)i * nout * 20, &na);
+      if (seconds) seconds[i] = std::chrono::duration<double>(std::chrono::steady_clock::now() - clk).count();
       naccept[i] = na;
     }
   };

@@ -114,9 +114,10 @@ class Problem:
         ns = np.zeros(nk, dtype=np.int32)
         na = np.zeros(nk, dtype=np.int32)
         ts = np.zeros(nk)
+        sec = np.zeros(nk)
         lib().oracle_evolve(*self._common(), ctypes.c_int(nk), _p(kmodes), ctypes.c_int(nout), _p(tau_out), _p(y),
-                            _p(st, _ip), _p(ns, _ip), _p(na, _ip), _p(ts), ctypes.c_int(nthreads))
-        return dict(y=y, status=st, nsteps=ns, naccept=na, tau_start=ts)
+                            _p(st, _ip), _p(ns, _ip), _p(na, _ip), _p(ts), ctypes.c_int(nthreads), _p(sec))
+        return dict(y=y, status=st, nsteps=ns, naccept=na, tau_start=ts, seconds=sec)
 
     def evolve_batched(self, kmodes, tau_out, batch: int, nthreads: int = 1):
         """evolve_modes_batched: groups of `batch` consecutive k share one step sequence."""
@@ -134,6 +135,26 @@ class Problem:
         if r != 0:
             raise ValueError("num_k must be a multiple of batch")
         return dict(y=y, status=st, nsteps=ns, naccept=na)
+
+    def step_control(self, k, err, y0, y1, prev_inv=1.0, prev_prev_inv=1.0):
+        """(norm, accepted, factor, inverse norm carried on) of one attempted step."""
+        err = np.ascontiguousarray(err, dtype=np.float64)
+        y0 = np.ascontiguousarray(y0, dtype=np.float64)
+        y1 = np.ascontiguousarray(y1, dtype=np.float64)
+        out = np.zeros(4)
+        lib().oracle_step_control(ctypes.byref(self.cfg), ctypes.c_double(k), _p(err), _p(y0), _p(y1),
+                                  ctypes.c_double(prev_inv), ctypes.c_double(prev_prev_inv), _p(out))
+        return out[0], bool(out[1]), out[2], out[3]
+
+    def batch_trace(self, kmodes, tau_out, max_trace: int = 8):
+        """The modes ``kmodes`` as one group of evolve_modes_batched: (attempted
+        steps, trace[i] = (t0, t1, group error norm) of the first steps)."""
+        kmodes = np.ascontiguousarray(kmodes, dtype=np.float64)
+        tau_out = np.ascontiguousarray(np.atleast_1d(tau_out), dtype=np.float64)
+        tr = np.zeros((max_trace, 3))
+        ns = lib().oracle_batch_trace(*self._common(), ctypes.c_int(kmodes.size), _p(kmodes),
+                                      ctypes.c_int(tau_out.size), _p(tau_out), ctypes.c_int(max_trace), _p(tr))
+        return ns, tr
 
     # ---- tangents along one input direction (dcos[24], dtab[stride]) ----
     def _tan(self, dcos, dtab):
@@ -184,10 +205,11 @@ class Problem:
         y, dy = np.zeros((nk, nout, 20)), np.zeros((nk, nout, 20))
         st = np.zeros(nk, dtype=np.int32)
         na = np.zeros(nk, dtype=np.int32)
+        sec = np.zeros(nk)
         lib().oracle_evolve_tangent(*self._tan(dcos, dtab), ctypes.c_int(nk), _p(kmodes), ctypes.c_int(nout),
                                     _p(tau_out), _p(dtau_out), _p(y), _p(dy), _p(st, _ip), _p(na, _ip),
-                                    ctypes.c_int(nthreads))
-        return dict(y=y, dy=dy, status=st, naccept=na)
+                                    ctypes.c_int(nthreads), _p(sec))
+        return dict(y=y, dy=dy, status=st, naccept=na, seconds=sec)
 
     def replay_tangent(self, k, times, tau_out, dtau_out, dcos, dtab):
         """Output fields and their tangent for the accepted step end times

@@ -1,0 +1,136 @@
+"""Writes the per-configuration golden vectors ``golden_<config>.json`` from
+the oracle (oracle/deb_oracle.cpp): the reference's evolve_perturbations path
+restated on the CPU.
+
+For representative k of each benchmark configuration: the k grid, tau_out,
+the 20 output fields at every output, P(k) at the last output, the start time
+and the attempted / accepted step counts, at rtol 1e-5 (atol 1e-8) and -- for
+k <= 0.1 /Mpc, where the oracle finishes in seconds -- at rtol 1e-10
+(atol 1e-14), and P(k) at rtol 1e-6 (the result's sensitivity to the
+tolerance).  The gradient configuration adds dP/dtheta by central
+differences (relative step 1e-4) of rtol 1e-10 solves.
+
+    python tests/golden/make_golden.py [config ...]
+"""
+from __future__ import annotations
+
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path[:0] = [os.path.join(ROOT, "disco-eb_amd"), os.path.join(ROOT, "oracle")]
+
+import oracle  # noqa: E402
+from discoeb_amd import packing  # noqa: E402
+from discoeb_amd.background import evolve_background  # noqa: E402
+from discoeb_amd.cosmology import base_theta, loss_weights, param_from_theta, sample_thetas, SEED  # noqa: E402
+
+LOOSE = (1e-5, 1e-8)
+MID = (1e-6, 1e-9)
+TIGHT = (1e-10, 1e-14)
+LMAX32 = dict(lmaxg=32, lmaxgp=32, lmaxr=32, lmaxnu=17, nqmax=5)
+FD_DIRS = (0, 1, 2, 5, 6, 7)
+FD_REL = 1e-4
+
+CONFIGS = {
+    # flat LCDM, no massive nu, lmax 16 (the reference's default nqmax = 3 leaves
+    # an inert massive-nu block), 64 k in [1e-4, 1], a = 1
+    "c1-lcdm-lmax16": dict(thetas=[base_theta(w0=-1.0, wa=0.0)], massive_nu=False,
+                           layout=dict(lmaxg=16, lmaxgp=16, lmaxr=16, lmaxnu=8, nqmax=3),
+                           kgrid=(1e-4, 1.0, 64), kidx=[0, 9, 18, 27, 36, 45, 54, 63], aexp=[1.0]),
+    # base cosmology + one massive nu (0.06 eV), w0 -0.9, wa 0.1, lmax 32, 16 outputs
+    "c2-mnu-w0wa-lmax32": dict(thetas=[base_theta()], massive_nu=True, layout=LMAX32,
+                               kgrid=(1e-5, 10.0, 256), kidx=[0, 32, 64, 96, 128, 160, 192, 224, 255],
+                               aexp=list(np.geomspace(0.01, 1.0, 16))),
+    # seeded box draws 0-4 (0 and 3 cross w = -1, 2 and 4 stay phantom), lmax 32
+    "c3-box-lmax32": dict(thetas=list(sample_thetas(64, SEED)[:5]), massive_nu=True, layout=LMAX32,
+                          kgrid=(1e-5, 10.0, 256), kidx=[0, 64, 128, 192, 255], aexp=[1.0]),
+    # the base cosmology's P(k) at a = 1 and dP/dtheta by central differences
+    "c4-grad-base": dict(thetas=[base_theta()], massive_nu=True, layout=LMAX32, kgrid=(1e-5, 10.0, 256),
+                         kidx=[0, 64, 128, 160], aexp=[1.0], fd=True),
+    # lmax 50, two of the 256 seeded draws, 512 k in [1e-5, 20]; loose tolerance only
+    "c5-box-lmax50": dict(thetas=list(sample_thetas(256, SEED)[:2]), massive_nu=True,
+                          layout=dict(lmaxg=50, lmaxgp=50, lmaxr=50, lmaxnu=17, nqmax=5),
+                          kgrid=(1e-5, 20.0, 512), kidx=[0, 128, 256, 384, 511], aexp=[1.0], tight=False),
+}
+
+
+def power(theta, k, dm):
+    A_s = np.exp(theta[4]) / 1e10
+    return 2 * np.pi**2 * A_s * (k / 0.05) ** (theta[3] - 1) * k ** (-3) * dm**2
+
+
+def solve(param, b, layout, k, tau_out, tol):
+    cfg = packing.make_config(rtol=tol[0], atol=tol[1], max_steps=2_000_000, **layout)
+    prob = oracle.Problem(param, cfg, b=b)
+    return prob.evolve(k, tau_out, nthreads=min(8, k.size))
+
+
+def entry(r):
+    return dict(y=r["y"].tolist(), tau_start=r["tau_start"].tolist(), nsteps=r["nsteps"].tolist(),
+                naccept=r["naccept"].tolist(), status=r["status"].tolist())
+
+
+def make(name):
+    c = CONFIGS[name]
+    thetas = np.array(c["thetas"])
+    with np.errstate(all="ignore"):
+        param = evolve_background(param=param_from_theta(thetas, c["massive_nu"]))
+    kall = np.geomspace(*c["kgrid"])
+    k = kall[c["kidx"]]
+    tau = packing.tau_of_aexp(param, c["aexp"])
+    out = dict(config=name, thetas=thetas.tolist(), massive_nu=c["massive_nu"], layout=c["layout"],
+               kgrid=list(c["kgrid"]), kidx=c["kidx"], k=k.tolist(), aexp=list(map(float, c["aexp"])),
+               tau_out=tau.tolist(), loose=dict(rtol=LOOSE[0], atol=LOOSE[1]),
+               tight=dict(rtol=TIGHT[0], atol=TIGHT[1]), cosmologies=[])
+    tight_sel = np.where(k <= 0.1)[0] if c.get("tight", True) else np.zeros(0, dtype=int)
+    for b, th in enumerate(thetas):
+        t0 = time.time()
+        rl = solve(param, b, c["layout"], k, tau[b], LOOSE)
+        # the same solve at rtol 1e-6: how much the result moves with the
+        # tolerance (large where w crosses -1, where the dark-energy equations
+        # carry c_s^2 / (1 + w))
+        rm = solve(param, b, c["layout"], k, tau[b], MID)
+        e = dict(loose=entry(rl), P=power(th, k, rl["y"][:, -1, 4]).tolist(),
+                 P_mid=power(th, k, rm["y"][:, -1, 4]).tolist(), tight_kidx=tight_sel.tolist())
+        if tight_sel.size:
+            rt = solve(param, b, c["layout"], k[tight_sel], tau[b], TIGHT)
+            e["tight"] = entry(rt)
+            e["P_tight"] = power(th, k[tight_sel], rt["y"][:, -1, 4]).tolist()
+        if c.get("fd"):
+            e["dP_dtheta_fd"] = fd_gradient(th, c, k[tight_sel]).tolist()
+            e["fd_dirs"] = list(FD_DIRS)
+        out["cosmologies"].append(e)
+        print(f"{name}: cosmology {b} done ({time.time() - t0:.0f} s)", flush=True)
+    if name == "c5-box-lmax50":
+        # loss weights w(k) ~ U[0, 1] from the same generator after the 256 draws
+        out["loss_weights"] = loss_weights(512, 256, SEED)[c["kidx"]].tolist()
+    with open(os.path.join(HERE, f"golden_{name}.json"), "w") as f:
+        json.dump(out, f, separators=(",", ":"))
+
+
+def fd_gradient(theta, c, k):
+    """dP/dtheta_j for the solve directions by central differences of rtol 1e-10 solves."""
+    g = np.zeros((k.size, len(FD_DIRS)))
+    for j, d in enumerate(FD_DIRS):
+        Ps = []
+        for sgn in (+1.0, -1.0):
+            th = theta.copy()
+            th[d] = th[d] * (1.0 + sgn * FD_REL)
+            with np.errstate(all="ignore"):
+                p = evolve_background(param=param_from_theta(th[None], c["massive_nu"]))
+            tau = packing.tau_of_aexp(p, c["aexp"])[0]
+            r = solve(p, 0, c["layout"], k, tau, TIGHT)
+            Ps.append(power(th, k, r["y"][:, -1, 4]))
+        g[:, j] = (Ps[0] - Ps[1]) / (2 * FD_REL * theta[d])
+    return g
+
+
+if __name__ == "__main__":
+    for name in (sys.argv[1:] or list(CONFIGS)):
+        make(name)

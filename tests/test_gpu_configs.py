@@ -1,0 +1,285 @@
+"""GPU parity per benchmark configuration, against the oracle's golden vectors
+(tests/golden/golden_<config>.json, written by tests/golden/make_golden.py).
+
+  c1  flat LCDM with a cosmological constant, no massive nu, lmax 16, k in [1e-4, 1]
+  c2  base cosmology + one massive nu, w0 -0.9, wa 0.1, lmax 32, 16 outputs in [0.01, 1]
+  c3  seeded box draws 0-4 (two cross w = -1), lmax 32
+  c4  the base cosmology of the gradient configuration
+  c5  two seeded lmax-50 draws, k up to 20 /Mpc
+
+Each test compares the GPU solve (through the C ABI) with the oracle at the
+same settings; at rtol 1e-5 the two implementations round differently, which
+can flip an accept/reject decision, so they agree at the tolerance level
+there and at the solution's own tolerance sensitivity at rtol 1e-10.  The
+measured differences are printed (pytest -s) and recorded in DESIGN.md.
+"""
+import ctypes
+import json
+import os
+
+import numpy as np
+import pytest
+
+from discoeb_amd import _lib, packing
+from discoeb_amd.background import evolve_background
+from discoeb_amd.cosmology import param_from_theta
+from discoeb_amd.perturbations import DeviceInputs, solve_forward
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+NAMES = ["c1-lcdm-lmax16", "c2-mnu-w0wa-lmax32", "c3-box-lmax32", "c4-grad-base", "c5-box-lmax50"]
+DELTAS = (4, 6, 8, 10, 12, 14)  # delta_m, delta_bc, delta_c, delta_b, delta_gamma, delta_r
+
+
+def _golden(name):
+    with open(os.path.join(GOLDEN, f"golden_{name}.json")) as f:
+        return json.load(f)
+
+
+_PARAMS = {}
+
+
+def _param(g):
+    if g["config"] not in _PARAMS:
+        with np.errstate(all="ignore"):
+            _PARAMS[g["config"]] = evolve_background(param=param_from_theta(np.array(g["thetas"]), g["massive_nu"]))
+    return _PARAMS[g["config"]]
+
+
+def _gpu(g, tol, kk=None, return_full=False):
+    import torch
+    param = _param(g)
+    cfg = packing.make_config(rtol=tol["rtol"], atol=tol["atol"], max_steps=2_000_000, return_full=return_full,
+                              **g["layout"])
+    k = np.asarray(g["k"] if kk is None else kk)
+    B = len(g["thetas"])
+    inp = DeviceInputs(param, cfg)
+    tau = np.asarray(g["tau_out"])
+    y, st, ns = solve_forward(inp, torch.from_numpy(np.ascontiguousarray(np.broadcast_to(k, (B, k.size)))),
+                              torch.from_numpy(tau), throw=False)
+    return y.cpu().numpy(), st.cpu().numpy(), ns.cpu().numpy()
+
+
+def _power(theta, k, dm):
+    return 2 * np.pi**2 * np.exp(theta[4]) / 1e10 * (k / 0.05) ** (theta[3] - 1) * k ** (-3) * dm**2
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_start_times_match_oracle(gpu, name):
+    """determine_starting_time on the GPU (deb_start_times) equals the oracle's,
+    which the dense-scan test pins (tests/test_oracle_pins.py)."""
+    import torch
+    g = _golden(name)
+    param = _param(g)
+    cfg = packing.make_config(**g["layout"])
+    inp = DeviceInputs(param, cfg)
+    B, k = len(g["thetas"]), np.asarray(g["k"])
+    kk = torch.from_numpy(np.ascontiguousarray(np.broadcast_to(k, (B, k.size)))).to(inp.device)
+    tau = torch.from_numpy(np.asarray(g["tau_out"])).to(inp.device)
+    ts = torch.zeros((B, k.size), dtype=torch.float64, device=inp.device)
+    L = _lib.lib()
+    _lib.check(L.deb_start_times(ctypes.byref(cfg), B, _lib.ptr(inp.cosmo), _lib.ptr(inp.tables), inp.stride,
+                                 _lib.ptr(inp.desc), _lib.ptr(inp.q), _lib.ptr(inp.w), k.size, _lib.ptr(kk),
+                                 tau.shape[1], _lib.ptr(tau), _lib.ptr(ts),
+                                 ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)), "deb_start_times")
+    got = ts.cpu().numpy()
+    for b, c in enumerate(g["cosmologies"]):
+        want = np.asarray(c["loose"]["tau_start"])
+        np.testing.assert_allclose(got[b], want, rtol=1e-13, err_msg=name)
+
+
+def _crosses(theta):
+    """w(a) = w0 + wa (1 - a) crosses -1 for some a in (0, 1]."""
+    return (theta[6] + 1.0) * (theta[6] + theta[7] + 1.0) < 0.0
+
+
+MATTER = (4, 6, 8, 10)       # per-mode relative comparisons
+RADIATION = (12, 14)         # decayed by a = 1 at high k: relative to the largest |value| over k
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_config_matches_oracle_at_rtol_1e5(gpu, name):
+    """rtol 1e-5, atol 1e-8: every mode OK, step counts within 6 % of the
+    oracle's (rounding differences flip accept/reject decisions; 5.2 % measured
+    at k = 1 of config 1), final matter fields (step end points) within 30 rtol
+    per mode, photon and massless-nu densities within 30 rtol of their largest
+    value over k, and P(k) within 3e-5 -- or, where the result moves more than
+    that with the tolerance itself, within 5 times that move (|P(rtol 1e-5) /
+    P(rtol 1e-6) - 1| of the oracle).  That happens where w(a) crosses -1: the
+    dark-energy equations carry c_s^2 / (1 + w) (perturbations.py:364-369), a
+    singular coefficient at the crossing, and the solution depends on where the
+    steps land around it."""
+    g = _golden(name)
+    y, st, ns = _gpu(g, g["loose"])
+    assert np.all(st == 0), st
+    worst_P, worst_f, worst_sens = 0.0, 0.0, 0.0
+    k = np.asarray(g["k"])
+    for b, c in enumerate(g["cosmologies"]):
+        ref = np.asarray(c["loose"]["y"])
+        assert np.all(np.asarray(c["loose"]["status"]) == 0)
+        nref = np.asarray(c["loose"]["nsteps"])
+        assert np.all(np.abs(ns[b] - nref) <= 0.06 * nref + 2), (ns[b], nref)
+        th = np.asarray(g["thetas"][b])
+        P_ref, P_mid = np.asarray(c["P"]), np.asarray(c["P_mid"])
+        sens = np.abs(P_mid / P_ref - 1)
+        P = _power(th, k, y[b, :, -1, 4])
+        errP = np.abs(P / P_ref - 1)
+        bound = np.maximum(3e-5, 5 * sens)
+        assert np.all(errP <= bound), (name, b, _crosses(th), errP, sens)
+        worst_P = max(worst_P, float(errP.max()))
+        worst_sens = max(worst_sens, float(sens.max()))
+        fbound = np.maximum(30e-5, 5 * sens)
+        for f in MATTER:
+            rel = np.abs(y[b, :, -1, f] - ref[:, -1, f]) / (np.abs(ref[:, -1, f]) + 1e-300)
+            assert np.all(rel <= fbound), (name, b, f, rel)
+            worst_f = max(worst_f, float(rel.max()))
+        for f in RADIATION:
+            rel = np.abs(y[b, :, -1, f] - ref[:, -1, f]) / np.abs(ref[:, -1, f]).max()
+            assert np.all(rel <= fbound), (name, b, f, rel)
+        if name.startswith("c1"):
+            assert np.all(y[b, :, :, 18:20] == 0.0)  # Lambda: delta_Q = theta_Q = 0
+    print(f"{name}: rtol 1e-5 GPU vs oracle: P(k) max rel diff {worst_P:.2e} (oracle's own rtol 1e-5 -> 1e-6 "
+          f"move {worst_sens:.2e}), final matter fields {worst_f:.2e}")
+
+
+@pytest.mark.parametrize("name", [n for n in NAMES if not n.startswith("c5")])
+def test_config_matches_oracle_at_rtol_1e10(gpu, name):
+    """rtol 1e-10, atol 1e-14 on the modes with k <= 0.1: final outputs within
+    1e-8 of the field's scale (delta_gamma, delta_r 2e-6 of their largest value
+    over k: those fields have decayed by orders of magnitude by a = 1), P(k)
+    within 1e-8.  Draws whose w(a) crosses -1 are left out: there the oracle
+    does not converge either (delta_m at k = 1e-5 moves by 5e-5 between rtol
+    1e-10 and 1e-12; their rtol 1e-5 comparison above bounds them by that
+    move).  Also states the rtol 1e-5 GPU result's error against this tight
+    solution (the error of the benchmark's settings)."""
+    g = _golden(name)
+    sel = np.asarray(g["cosmologies"][0]["tight_kidx"])
+    k = np.asarray(g["k"])[sel]
+    y, st, _ = _gpu(g, g["tight"], kk=k)
+    yl, _, _ = _gpu(g, g["loose"], kk=k)
+    assert np.all(st == 0)
+    err_loose, checked = 0.0, 0
+    for b, c in enumerate(g["cosmologies"]):
+        th = np.asarray(g["thetas"][b])
+        if _crosses(th):
+            continue
+        checked += 1
+        ref = np.asarray(c["tight"]["y"])
+        for f in MATTER + (0, 1, 2, 3):
+            rel = np.abs(y[b, :, -1, f] - ref[:, -1, f]) / (np.abs(ref[:, -1, f]) + 1e-300)
+            assert rel.max() < 1e-8, (name, b, f, rel.max())
+        for f in RADIATION:
+            rel = np.abs(y[b, :, -1, f] - ref[:, -1, f]) / np.abs(ref[:, -1, f]).max()
+            assert rel.max() < 2e-6, (name, b, f, rel.max())
+        P = _power(th, k, y[b, :, -1, 4])
+        assert np.abs(P / np.asarray(c["P_tight"]) - 1).max() < 1e-8
+        err_loose = max(err_loose, float(np.abs(_power(th, k, yl[b, :, -1, 4]) / np.asarray(c["P_tight"]) - 1).max()))
+    assert checked >= 1
+    print(f"{name}: P(k) error of the rtol 1e-5 GPU solve against rtol 1e-10: {err_loose:.2e}")
+
+
+def test_c2_all_output_fields(gpu):
+    """Config 2: all 20 fields at all 16 outputs.  Final outputs (a step end)
+    within 30 rtol -- per mode for the matter, metric and neutrino fields, of
+    the largest |value| over k for the photon, massless-nu and dark-energy
+    fields (decayed at high k by late times); theta_c vanishes identically
+    (the oracle's dense LU leaves ~1e-11 of rounding there, the structured
+    solve exact zeros).  The 15 interpolated outputs move with the step end
+    points (linear SaveAt interpolation): 2e-3."""
+    g = _golden("c2-mnu-w0wa-lmax32")
+    y, st, _ = _gpu(g, g["loose"])
+    assert np.all(st == 0)
+    ref = np.asarray(g["cosmologies"][0]["loose"]["y"])
+    worst_final, worst_all = 0.0, 0.0
+    for f in range(20):
+        if f == 9:
+            big = np.abs(ref[:, :, 11]).max()
+            assert np.abs(y[0, :, :, 9]).max() <= 1e-9 * big and np.abs(ref[:, :, 9]).max() <= 1e-9 * big
+            continue
+        if f in (12, 13, 14, 15, 18, 19):
+            sc = np.abs(ref[:, :, f]).max(axis=0, keepdims=True) + 1e-300
+        else:
+            sc = np.abs(ref[:, :, f]) + 1e-300
+        rel = np.abs(y[0, :, :, f] - ref[:, :, f]) / sc
+        worst_final = max(worst_final, float(rel[:, -1].max()))
+        worst_all = max(worst_all, float(rel.max()))
+        assert rel[:, -1].max() < 30 * 1e-5, (f, rel[:, -1].max())
+        assert rel.max() < 2e-3, (f, rel.max())
+    print(f"c2 all 20 fields: final {worst_final:.2e}, all outputs {worst_all:.2e}")
+
+
+def test_return_full_matches_oracle(gpu, test_param):
+    """return_full=True (perturbations.py:773-779): the raw state at every
+    output, against the oracle at rtol 1e-10."""
+    import oracle
+    import torch
+    cfg = packing.make_config(lmaxg=12, lmaxgp=12, lmaxr=12, lmaxnu=8, nqmax=3, rtol=1e-10, atol=1e-14,
+                              max_steps=400000, return_full=True)
+    k = np.geomspace(1e-3, 0.3, 4)
+    tau = packing.tau_of_aexp(test_param, [0.1, 1.0])[0]
+    ref = oracle.Problem(test_param, cfg).evolve(k, tau, nthreads=4)
+    inp = DeviceInputs(test_param, cfg)
+    y, st, _ = solve_forward(inp, torch.from_numpy(k[None]), torch.from_numpy(tau[None]), throw=False)
+    y = y[0].cpu().numpy()
+    n = packing.nvar(12, 12, 12, 8, 3)
+    assert y.shape == (4, 2, n) and np.all(st.cpu().numpy() == 0)
+    r = ref["y"]
+    # every entry at the final output within 1e-6 of its largest |value| over k
+    # (floored at 1e-8 of the state's largest entry: theta_c vanishes identically,
+    # where the oracle's dense LU leaves rounding and the structured solve zeros)
+    sc = np.maximum(np.abs(r[:, -1, :]).max(axis=0, keepdims=True), 1e-8 * np.abs(r[:, -1, 2:]).max())
+    assert (np.abs(y[:, -1] - r[:, -1]) / sc).max() < 1e-6
+    for idx in (0, 2, 3, 5, 7):
+        assert np.abs(y[:, -1, idx] / r[:, -1, idx] - 1).max() < 1e-8
+
+
+def test_nonfinite_tables_fail_the_mode_with_inf_outputs(gpu, test_param):
+    """A NaN knot in the opacity table: the steps are rejected until the mode
+    ends (NONFINITE or MAX_STEPS), the unreached outputs hold inf, and
+    throw=True raises (diffrax's throw)."""
+    import torch
+    cfg = packing.make_config(lmaxg=11, lmaxgp=11, lmaxr=11, lmaxnu=8, nqmax=3, rtol=1e-4, atol=1e-8,
+                              max_steps=3000)
+    inp = DeviceInputs(test_param, cfg)
+    off, nn = inp.desc[packing.SPLINE_KEYS.index("xe_of_loga_spline")]
+    inp.tables[0, int(off) + int(nn) + int(nn) // 2] = float("nan")
+    k = np.array([0.05, 0.2])
+    tau = packing.tau_of_aexp(test_param, [1e-4, 1.0])[0]
+    y, st, _ = solve_forward(inp, torch.from_numpy(k[None]), torch.from_numpy(tau[None]), throw=False)
+    st = st.cpu().numpy()
+    assert np.all((st == packing.STATUS_MAX_STEPS) | (st == packing.STATUS_NONFINITE)), st
+    assert np.all(np.isinf(y[0, :, -1].cpu().numpy()))
+    with pytest.raises(_lib.DebError):
+        solve_forward(inp, torch.from_numpy(k[None]), torch.from_numpy(tau[None]), throw=True)
+
+
+def test_c3_full_size_launch(gpu):
+    """Config 3 at full size: 64 seeded cosmologies x 256 k in one launch, every
+    mode OK and finite; two cosmologies launched alone give bit-identical
+    results (a mode's result does not depend on which others share the launch)."""
+    import torch
+    from discoeb_amd.cosmology import SEED, sample_thetas
+    th = sample_thetas(64, SEED)
+    with np.errstate(all="ignore"):
+        param = evolve_background(param=param_from_theta(th))
+    cfg = packing.make_config(lmaxg=32, lmaxgp=32, lmaxr=32, lmaxnu=17, nqmax=5, rtol=1e-5, atol=1e-8,
+                              max_steps=32768)
+    k = np.geomspace(1e-5, 10.0, 256)
+    inp = DeviceInputs(param, cfg)
+    tau = packing.tau_of_aexp(param, [1.0])
+    y, st, _ = solve_forward(inp, torch.from_numpy(np.ascontiguousarray(np.broadcast_to(k, (64, 256)))),
+                             torch.from_numpy(tau), throw=False)
+    assert bool((st == 0).all()) and bool(torch.isfinite(y).all())
+    for b in (0, 37):
+        sub = {kk: (v[b:b + 1] if isinstance(v, np.ndarray) and v.ndim == 1 and v.shape[0] == 64 else v)
+               for kk, v in param.items() if not hasattr(v, "evaluate")}
+        for kk, v in param.items():
+            if hasattr(v, "evaluate"):
+                import copy
+                s = copy.copy(v)
+                s.x, s.y, s.S = v.x[b:b + 1], v.y[b:b + 1], v.S[b:b + 1]
+                sub[kk] = s
+        inp1 = DeviceInputs(sub, cfg)
+        y1, st1, _ = solve_forward(inp1, torch.from_numpy(k[None]), torch.from_numpy(tau[b:b + 1]), throw=False)
+        assert torch.equal(y[b], y1[0]) and torch.equal(st[b], st1[0])

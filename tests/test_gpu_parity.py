@@ -222,8 +222,16 @@ def _slice_spline(sp, b):
 
 
 def test_max_steps_status_and_throw(gpu, test_param):
+    """max_steps reached: with throw=False the outputs the modes never reached
+    hold inf (diffrax's unreached SaveAt entries); with throw=True it raises."""
     from discoeb_amd.perturbations import evolve_perturbations
-    y, k, p = evolve_perturbations(param=test_param, aexp_out=[1.0], kmin=1e-3, kmax=1.0, num_k=4, max_steps=3,
-                                   throw=False)
+    y, k, p = evolve_perturbations(param=test_param, aexp_out=[0.5, 1.0], kmin=1e-3, kmax=1.0, num_k=4,
+                                   max_steps=3, throw=False)
+    assert bool(torch_isinf_all(y))
     with pytest.raises(_lib.DebError):
         evolve_perturbations(param=test_param, aexp_out=[1.0], kmin=1e-3, kmax=1.0, num_k=4, max_steps=3)
+
+
+def torch_isinf_all(y):
+    import torch
+    return torch.isinf(y).all()
