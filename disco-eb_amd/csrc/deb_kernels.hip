@@ -18,6 +18,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -27,6 +28,38 @@
 namespace deb {
 
 thread_local std::string g_err;
+
+// Workspace of deb_solve_forward_batched_noalloc: a file-scope device array
+// (each GPU the library runs on holds its own copy until the process ends).
+// Host code never passes `scratch` itself to a kernel (in host code the name
+// is a stand-in, not a device address): scratch_ptr() returns the device
+// address from hipGetSymbolAddress, looked up once per device.
+constexpr size_t SCRATCH_BYTES = 64ull << 20;
+__device__ __align__(256) double scratch[SCRATCH_BYTES / sizeof(double)];
+
+constexpr int MAX_DEVICES = 64;
+static std::mutex g_scratch_mutex;
+static void *g_scratch_ptr[MAX_DEVICES];
+
+// the current device's address of `scratch` (nullptr and an error message on failure)
+static void *scratch_ptr() {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEVICES) {
+    fail(DEB_E_HIP, "no current HIP device for the allocation-free workspace");
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      fail(DEB_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+      return nullptr;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  return g_scratch_ptr[dev];
+}
 
 // evolve_perturbations_batched: one wavefront per (cosmology, group of B
 // consecutive k), groups of the largest k first.
@@ -281,6 +314,30 @@ int deb_solve_forward_batched(const deb_config_t *cfg, int n_cosmo, const deb_co
   hipStream_t st = (hipStream_t)stream;
   DEB_DISPATCH(L.ns, (run_forward_batched<NS>(cfg, L, n_cosmo, cosmo, tables, table_stride, desc, q, w, nk, kmodes,
                                               nout, tau_out, batch_size, y_out, status, nsteps, workspace, st)))
+}
+
+int deb_solve_forward_batched_noalloc(const deb_config_t *cfg, int n_cosmo, const deb_cosmo_t *cosmo,
+                                      const double *tables, int64_t table_stride, const int32_t *desc,
+                                      const double *q, const double *w, int nk, const double *kmodes, int nout,
+                                      const double *tau_out, int batch_size, double *y_out, int32_t *status,
+                                      int32_t *nsteps, void *stream) {
+  if (int r = check_common(cfg, n_cosmo, cosmo, tables, desc, q, w)) return r;
+  if (nk < 1 || nout < 1 || !kmodes || !tau_out || !y_out || !status) return fail(DEB_E_ARG, "invalid mode arrays");
+  if (batch_size < 1 || nk % batch_size != 0) return fail(DEB_E_ARG, "num_k must be a multiple of batch_size");
+  if ((int64_t)n_cosmo * nk > 0x7fffffff) return fail(DEB_E_ARG, "too many modes");
+  size_t need = 0;
+  if (int r = deb_batched_workspace_bytes(cfg, n_cosmo, nk, &need)) return r;
+  if (need > SCRATCH_BYTES)
+    return fail(DEB_E_ARG, "n_cosmo x num_k needs " + std::to_string(need) +
+                               " workspace bytes; the allocation-free entry point holds " +
+                               std::to_string(SCRATCH_BYTES) + " (use deb_solve_forward_batched)");
+  void *ws = scratch_ptr();
+  if (!ws) return DEB_E_HIP;
+  Layout L;
+  if (int r = build_layout(*cfg, L)) return r;
+  hipStream_t st = (hipStream_t)stream;
+  DEB_DISPATCH(L.ns, (run_forward_batched<NS>(cfg, L, n_cosmo, cosmo, tables, table_stride, desc, q, w, nk, kmodes,
+                                              nout, tau_out, batch_size, y_out, status, nsteps, ws, st)))
 }
 
 int deb_solve_forward_tape(const deb_config_t *cfg, int n_cosmo, const deb_cosmo_t *cosmo, const double *tables,

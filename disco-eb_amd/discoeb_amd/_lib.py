@@ -50,6 +50,9 @@ def lib():
         L.deb_solve_forward_batched.argtypes = [ctypes.POINTER(DebConfig), ctypes.c_int, _dp, _dp, _i64, _dp, _dp,
                                                 _dp, ctypes.c_int, _dp, ctypes.c_int, _dp, ctypes.c_int, _dp, _dp,
                                                 _dp, _dp, ctypes.c_size_t, _dp]
+        L.deb_solve_forward_batched_noalloc.argtypes = [ctypes.POINTER(DebConfig), ctypes.c_int, _dp, _dp, _i64,
+                                                        _dp, _dp, _dp, ctypes.c_int, _dp, ctypes.c_int, _dp,
+                                                        ctypes.c_int, _dp, _dp, _dp, _dp]
         L.deb_tape_bytes.argtypes = [ctypes.POINTER(DebConfig), ctypes.c_int, ctypes.c_int, _i64, ctypes.c_int,
                                      ctypes.POINTER(ctypes.c_size_t)]
         L.deb_solve_forward_tape.argtypes = [ctypes.POINTER(DebConfig), ctypes.c_int, _dp, _dp, _i64, _dp, _dp, _dp,

@@ -155,6 +155,24 @@ int deb_solve_forward_batched(const deb_config_t *cfg, int n_cosmo, const deb_co
                               void *stream);
 
 /*
+ * deb_solve_forward_batched without a workspace argument: the two states per
+ * mode live in a 64 MiB device array inside the library (each GPU holds its
+ * own copy until the process ends), so the call allocates nothing.  No
+ * counterpart in the reference: it is for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops).  Limit:
+ * n_cosmo * nk * 2 * deb_slots * 512 bytes <= 64 MiB (DEB_E_ARG beyond; e.g.
+ * 64 cosmologies x 256 k at lmax 32).  Every call on one GPU shares that GPU's
+ * array: calls that overlap on one GPU (other streams, other host threads)
+ * race and must be serialised by the caller.  Arguments as
+ * deb_solve_forward_batched.
+ */
+int deb_solve_forward_batched_noalloc(const deb_config_t *cfg, int n_cosmo, const deb_cosmo_t *cosmo,
+                                      const double *tables, int64_t table_stride, const int32_t *desc,
+                                      const double *q, const double *w, int nk, const double *kmodes, int nout,
+                                      const double *tau_out, int batch_size, double *y_out, int32_t *status,
+                                      int32_t *nsteps, void *stream);
+
+/*
  * Test hook: the start time of every (cosmology, k) mode, as deb_solve_forward
  * computes it: determine_starting_time (perturbations.py:630-681, bisection
  * util.py:365-396) and tau_start = 0.99 min(min tau_out, tau*) (:743).
