@@ -134,8 +134,8 @@ __device__ __forceinline__ void bsolveT(const Mode<NS> &m, double (&x)[NS]) {
     const int s = 1 << L;
 #pragma unroll
     for (int j = 0; j < NS; ++j) {
-      const double v = m.al[L][j] * x[j];
-      const double w = m.be[L][j] * x[j];
+      const double v = m.al(L, j) * x[j];
+      const double w = m.be(L, j) * x[j];
       double vn, wp;
       if (L == 0) {
         vn = from_next(v);
@@ -1176,7 +1176,7 @@ __device__ __forceinline__ void output_vjp(const Params<NS> &P, Ctx &cx, const A
 // KBl, ls, lg: per-wave LDS scratch.
 template <int NS>
 __device__ __forceinline__ void adjoint_mode(const Params<NS> &P, const AdjArgs &A, int ci, int ki, double *ring,
-                                             double *KBl, double *ls, double *lg) {
+                                             double *KBl, double *ls, double *lg, lds_double *pcr) {
   const int lane = lane_id();
   const long long item = (long long)ci * P.nk + ki;
   double *gout = A.grad + item * A.ndir;
@@ -1191,6 +1191,7 @@ __device__ __forceinline__ void adjoint_mode(const Params<NS> &P, const AdjArgs 
   cx.dcos = A.dcos + (long long)ci * A.ndir * COSMO_DOUBLES;
   cx.dtab = A.dtab + (long long)ci * A.ndir * P.table_stride;
   Mode<NS> m;
+  m.pcr = pcr;
 #pragma unroll
   for (int s = 0; s < NS; ++s) m.rd[s] = P.rowdesc[s][lane];
   m.nlev = mode_levels(P.cfg, P.nq);

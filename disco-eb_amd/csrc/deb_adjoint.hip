@@ -28,9 +28,10 @@ namespace deb {
 // gradients instead of waiting.  The ring is released when the item is done.
 template <int NS>
 __global__ void __launch_bounds__(WAVE) deb_adjoint_kernel(const Params<NS> P, const AdjArgs A) {
+  // the stage cotangents of one reverse step; between steps the first two
+  // [NS][64] blocks stage the output conversion's reverse (output_vjp)
   __shared__ double KBl[7 * NS * WAVE];
-  __shared__ double ls[NS * WAVE];
-  __shared__ double lg[NS * WAVE];
+  __shared__ double pcr[pcr_lds_doubles<NS>()];
   const int item = blockIdx.x;
   if (item >= P.n_cosmo * P.nk) return;
   const int ki = P.nk - 1 - item / P.n_cosmo;
@@ -52,7 +53,8 @@ __global__ void __launch_bounds__(WAVE) deb_adjoint_kernel(const Params<NS> P, c
     if (lane < A.ndir) A.grad[((long long)ci * P.nk + ki) * A.ndir + lane] = __builtin_nan("");
     return;
   }
-  adjoint_mode<NS>(P, A, ci, ki, A.rings + (long long)ring * ring_doubles(NS, A.tape.ckpt), KBl, ls, lg);
+  adjoint_mode<NS>(P, A, ci, ki, A.rings + (long long)ring * ring_doubles(NS, A.tape.ckpt), KBl, KBl,
+                   KBl + NS * WAVE, (lds_double *)pcr);
   __threadfence();
   if (lane == 0) atomicExch(&A.ring_busy[ring], 0);
 }
@@ -130,12 +132,14 @@ template <int NS>
 __global__ void __launch_bounds__(WAVE) deb_step_vjp_kernel(const Params<NS> P, const VjpHook H) {
   __shared__ double Kl[8 * NS * WAVE];
   __shared__ double KBl[7 * NS * WAVE];
+  __shared__ double pcr[pcr_lds_doubles<NS>()];
   const int item = blockIdx.x;
   if (item >= H.m) return;
   const int lane = lane_id();
   Ctx cx;
   hook_ctx<NS>(cx, P, H, item);
   Mode<NS> m;
+  m.pcr = (lds_double *)pcr;
   m.nlev = mode_levels(P.cfg, P.nq);
   double y[NS], yb[NS];
   const double *ys = H.y + (long long)item * P.n, *ws = H.wbar + (long long)item * P.n;

@@ -25,6 +25,20 @@ constexpr int WAVE = 64;
 constexpr int MAXNS = 6;
 constexpr int PCR_LEVELS = 6;
 
+// pointers into a wave's LDS (address space 3: ds_read/ds_write, also inside
+// outlined functions, which would otherwise address LDS with flat instructions)
+#ifndef DEB_LDS
+#define DEB_LDS __attribute__((address_space(3)))
+#endif
+typedef DEB_LDS double lds_double;
+
+// doubles of LDS one wave needs for its step factors: the cyclic-reduction
+// coefficients (alpha, beta) of every level and row
+template <int NS>
+constexpr int pcr_lds_doubles() {
+  return PCR_LEVELS * 2 * NS * WAVE;
+}
+
 // row types (lane roles)
 enum : uint32_t {
   R_NONE = 0,

@@ -289,13 +289,14 @@ __device__ __forceinline__ int mode_levels(const deb_config_t &cf, int nq) {
 // evolve_one_mode (perturbations.py:728-781) for one (cosmology, k) item;
 // with TAPE every accepted state is also written to the checkpoint tape
 template <int NS, bool TAPE = false>
-__device__ __forceinline__ void integrate_mode(const Params<NS> &P, int ci, int ki, double *lds,
+__device__ __forceinline__ void integrate_mode(const Params<NS> &P, int ci, int ki, double *lds, lds_double *pcr,
                                                const TapeView *tape = nullptr) {
   const int lane = lane_id();
   Ctx cx;
   const double k = P.kmodes[(int64_t)ci * P.nk + ki];
   make_ctx<NS>(cx, P, ci, k);
   Mode<NS> m;
+  m.pcr = pcr;
 #pragma unroll
   for (int s = 0; s < NS; ++s) m.rd[s] = P.rowdesc[s][lane];
   m.nlev = mode_levels(P.cfg, P.nq);
@@ -403,12 +404,13 @@ __device__ __forceinline__ void integrate_mode(const Params<NS> &P, int ci, int 
 // state and proposal live in ws (2 * NS * WAVE doubles per mode, lane-owned).
 template <int NS>
 __device__ __forceinline__ void integrate_batch(const Params<NS> &P, int ci, int kb, int B, double *lds,
-                                                double *ws) {
+                                                lds_double *pcr, double *ws) {
   const int lane = lane_id();
   const int64_t mode0 = (int64_t)ci * P.nk + kb;
   Ctx cx;
   make_ctx<NS>(cx, P, ci, P.kmodes[mode0]);
   Mode<NS> m;
+  m.pcr = pcr;
 #pragma unroll
   for (int s = 0; s < NS; ++s) m.rd[s] = P.rowdesc[s][lane];
   m.nlev = mode_levels(P.cfg, P.nq);
@@ -538,11 +540,12 @@ __device__ __forceinline__ void integrate_batch(const Params<NS> &P, int ci, int
 template <int NS>
 __global__ void __launch_bounds__(WAVE) deb_forward_kernel(const Params<NS> P) {
   __shared__ double lds[WAVE * NS];
+  __shared__ double pcr[pcr_lds_doubles<NS>()];
   const int item = blockIdx.x;
   if (item >= P.n_cosmo * P.nk) return;
   const int ki = P.nk - 1 - item / P.n_cosmo;
   const int ci = item % P.n_cosmo;
-  integrate_mode<NS>(P, ci, ki, lds);
+  integrate_mode<NS>(P, ci, ki, lds, (lds_double *)pcr);
 }
 
 }  // namespace deb

@@ -66,12 +66,13 @@ static void *scratch_ptr() {
 template <int NS>
 __global__ void __launch_bounds__(WAVE) deb_forward_batched_kernel(const Params<NS> P, const int B, double *ws) {
   __shared__ double lds[WAVE * NS];
+  __shared__ double pcr[pcr_lds_doubles<NS>()];
   const int ng = P.nk / B;
   const int item = blockIdx.x;
   if (item >= P.n_cosmo * ng) return;
   const int g = ng - 1 - item / P.n_cosmo;
   const int ci = item % P.n_cosmo;
-  integrate_batch<NS>(P, ci, g * B, B, lds, ws);
+  integrate_batch<NS>(P, ci, g * B, B, lds, (lds_double *)pcr, ws);
 }
 
 // The forward solve that also writes every accepted state to the checkpoint
@@ -79,11 +80,12 @@ __global__ void __launch_bounds__(WAVE) deb_forward_batched_kernel(const Params<
 template <int NS>
 __global__ void __launch_bounds__(WAVE) deb_forward_tape_kernel(const Params<NS> P, const TapeView T) {
   __shared__ double lds[WAVE * NS];
+  __shared__ double pcr[pcr_lds_doubles<NS>()];
   const int item = blockIdx.x;
   if (item >= P.n_cosmo * P.nk) return;
   const int ki = P.nk - 1 - item / P.n_cosmo;
   const int ci = item % P.n_cosmo;
-  integrate_mode<NS, true>(P, ci, ki, lds, &T);
+  integrate_mode<NS, true>(P, ci, ki, lds, (lds_double *)pcr, &T);
 }
 
 // ---------------------------------------------------------------------------
@@ -120,12 +122,14 @@ __global__ void __launch_bounds__(WAVE) deb_rhs_kernel(const Params<NS> P, const
 
 template <int NS>
 __global__ void __launch_bounds__(WAVE) deb_step_kernel(const Params<NS> P, const HookArgs<NS> H) {
+  __shared__ double pcr[pcr_lds_doubles<NS>()];
   const int item = blockIdx.x;
   if (item >= H.m) return;
   const int lane = lane_id();
   Ctx cx;
   make_ctx<NS>(cx, P, H.ci[item], H.kk[item]);
   Mode<NS> m;
+  m.pcr = (lds_double *)pcr;
   {
     int maxlen = P.cfg.lmaxg + 2;
     maxlen = max(maxlen, P.cfg.lmaxgp + 1);

@@ -17,8 +17,13 @@ namespace deb {
 template <int NS>
 struct Mode {
   uint32_t rd[NS];
-  // PCR factors of W_B: d' = sig*d + al*d[i-s] + be*d[i+s] at level s = 2^L
-  double al[PCR_LEVELS][NS], be[PCR_LEVELS][NS];
+  // PCR factors of W_B: d' = sig*d + al*d[i-s] + be*d[i+s] at level s = 2^L;
+  // al and be live in the wave's LDS ([PCR_LEVELS][2][NS][64], pcr_lds_doubles)
+  lds_double *pcr;
+  __device__ __forceinline__ lds_double &al(int L, int j) const { return pcr[((L * 2) * NS + j) * WAVE + lane_id()]; }
+  __device__ __forceinline__ lds_double &be(int L, int j) const {
+    return pcr[((L * 2 + 1) * NS + j) * WAVE + lane_id()];
+  }
   double sig0[NS];
   double binv[NS];
   double Z[3][NS];   // W_B^{-1} [u_h, u_eta, u_pi]
@@ -137,7 +142,7 @@ __device__ __forceinline__ void bsolve(const Mode<NS> &m, double (&x)[NS]) {
         dp = from_lane(x[j], lp);
       }
       double sg = (L == 0) ? m.sig0[j] : 1.0;
-      x[j] = sg * x[j] + m.al[L][j] * dm + m.be[L][j] * dp;
+      x[j] = sg * x[j] + m.al(L, j) * dm + m.be(L, j) * dp;
     }
   }
 #pragma unroll
@@ -213,8 +218,8 @@ __device__ __forceinline__ void factor(Mode<NS> &m, Ctx &cx, double tau0, const 
       double nb = sg * Bd[j] + alpha * cm + beta * ap;
       double na = alpha * am;
       double nc = beta * cp;
-      m.al[L][j] = alpha;
-      m.be[L][j] = beta;
+      m.al(L, j) = alpha;
+      m.be(L, j) = beta;
       if (L == 0) m.sig0[j] = sg;
       Bd[j] = nb;
       A[j] = na;

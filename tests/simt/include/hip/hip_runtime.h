@@ -27,6 +27,8 @@
 // LDS arrays live in kernel bodies only, which the harness never runs (it calls
 // the per-item device functions with arrays it shares between a wave's threads)
 #define __shared__
+// LDS pointers are plain host pointers here
+#define DEB_LDS
 
 // kernel bodies refer to these; the harness never runs a kernel body
 struct simt_dim3 {

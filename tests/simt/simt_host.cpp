@@ -88,8 +88,8 @@ int run(const deb_config_t &cfg, const Layout &L, Reader &in, FILE *out, int mod
   P.nsteps = nsteps.data();
   if (mode == 0) {
     for (int ki = 0; ki < nk; ++ki) {
-      std::vector<double> lds(NS * WAVE, -777.0);
-      run_wave([&](int) { integrate_mode<NS>(P, 0, ki, lds.data()); });
+      std::vector<double> lds(NS * WAVE, -777.0), pcr(pcr_lds_doubles<NS>(), -444.0);
+      run_wave([&](int) { integrate_mode<NS>(P, 0, ki, lds.data(), pcr.data()); });
     }
     fwrite(status.data(), 4, nk, out);
     fwrite(nsteps.data(), 4, nk, out);
@@ -111,8 +111,8 @@ int run(const deb_config_t &cfg, const Layout &L, Reader &in, FILE *out, int mod
   TapeView tv;
   if (int r = tape_view(tv, tape.data(), tape_bytes, nk, NS, ckpt)) return r;
   for (int ki = 0; ki < nk; ++ki) {
-    std::vector<double> lds(NS * WAVE, -777.0);
-    run_wave([&](int) { integrate_mode<NS, true>(P, 0, ki, lds.data(), &tv); });
+    std::vector<double> lds(NS * WAVE, -777.0), pcr(pcr_lds_doubles<NS>(), -444.0);
+    run_wave([&](int) { integrate_mode<NS, true>(P, 0, ki, lds.data(), pcr.data(), &tv); });
   }
   AdjArgs A;
   std::memset(&A, 0, sizeof(A));
@@ -128,8 +128,12 @@ int run(const deb_config_t &cfg, const Layout &L, Reader &in, FILE *out, int mod
   A.n_in = output_inputs(cfg, A.in_idx);
   for (int ki = 0; ki < nk; ++ki) {
     std::vector<double> ring(ring_doubles(NS, ckpt), -555.0);
-    std::vector<double> KB(7 * NS * WAVE, -666.0), ls(NS * WAVE, -777.0), lg(NS * WAVE, -888.0);
-    run_wave([&](int) { adjoint_mode<NS>(P, A, 0, ki, ring.data(), KB.data(), ls.data(), lg.data()); });
+    std::vector<double> KB(7 * NS * WAVE, -666.0), pcr(pcr_lds_doubles<NS>(), -444.0);
+    // the output conversion's reverse stages its data in the first two blocks
+    // of the stage-cotangent buffer, as the kernel does
+    run_wave([&](int) {
+      adjoint_mode<NS>(P, A, 0, ki, ring.data(), KB.data(), KB.data(), KB.data() + NS * WAVE, pcr.data());
+    });
   }
   fwrite(status.data(), 4, nk, out);
   fwrite(nsteps.data(), 4, nk, out);
