@@ -12,12 +12,12 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "discoeb_amd", "libdiscoeb_hip.so")
-SOURCES = ["deb_kernels.hip", "deb_adjoint.hip"]
+SOURCES = ["deb_kernels.hip", "deb_adjoint.hip", "deb_observables.hip"]
 # Both units at -O3.  The reverse step's row passes are outlined device calls
 # (DEB_OUTLINE in deb_adjoint.h): builds with everything inlined into one
 # kernel body (thousands of spilled registers) gave build-dependent wrong
 # reverse steps at -O3 and at -O1.
-OPT = {"deb_kernels.hip": "-O3", "deb_adjoint.hip": "-O3"}
+OPT = {"deb_kernels.hip": "-O3", "deb_adjoint.hip": "-O3", "deb_observables.hip": "-O3"}
 HEADERS = ["deb_device.h", "deb_mode.h", "deb_solver.h", "deb_forward.h", "deb_host.h", "deb_adjoint.h"]
 FLAGS = ["--offload-arch=gfx950", "-std=c++17", "-fPIC",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

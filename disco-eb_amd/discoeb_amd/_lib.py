@@ -68,6 +68,16 @@ def lib():
         L.deb_rodas5_step_vjp.argtypes = [ctypes.POINTER(DebConfig), _dp, _dp, _i64, _dp, _dp, _dp, ctypes.c_int,
                                           _dp, _dp, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
                                           _dp]
+        _i = ctypes.c_int
+        L.deb_observables_scratch_bytes.restype = ctypes.c_size_t
+        L.deb_observables_scratch_bytes.argtypes = [_i, _i, _i]
+        L.deb_power.argtypes = [_i, _i, _dp, _dp, _i, _i, _dp, _dp, _dp, _dp, _dp]
+        L.deb_power_smoothed.argtypes = [_i, _i, _dp, _dp, _i, _i, _dp, _dp, _dp, _i, _i, _dp, ctypes.c_size_t, _dp,
+                                         _dp]
+        L.deb_power_kaiser.argtypes = [_i, _i, _i, _dp, _dp, _i, _dp, _dp, _dp, _dp, _dp, _i, _i, _dp,
+                                       ctypes.c_size_t, _dp, _dp]
+        L.deb_power_multipoles.argtypes = [_i, _i, _dp, _dp, _i, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.deb_xi_from_power.argtypes = [_i, _i, _dp, _dp, _i, _dp, _dp, _dp]
         if L.deb_abi_version() != 2:
             raise DebError("libdiscoeb_hip.so ABI version mismatch")
         _LIB = L

@@ -36,6 +36,7 @@ extern "C" {
 #define DEB_MAXNQ 16    /* maximum massive-neutrino momentum bins */
 #define DEB_NOUT_FIELDS 20
 #define DEB_MAXDIR 6    /* input tangent directions per gradient launch */
+#define DEB_OBS_MAXK 2048 /* k points per cosmology of the observables below */
 
 /* Spline tables read on the path (param dict keys of the reference). */
 enum {
@@ -290,6 +291,50 @@ int deb_rodas5_step_vjp(const deb_config_t *cfg, const deb_cosmo_t *cosmo, const
                         const double *dcosmo, const double *dtables, int m, const int32_t *ci, const double *kk,
                         const double *t0, const double *t1, const double *y0, const double *y1bar, double *y0bar,
                         double *t0bar, double *t1bar, double *grad, void *stream);
+
+/*
+ * Observables of the solution (one output time): y is device
+ * [n_cosmo][nk][nfield] (the fields of convert_to_output_variables; field 4 is
+ * delta_m, 5 theta_m/(aH)), k device [n_cosmo][nk], A_s, n_s, k_p, bias device
+ * [n_cosmo].  The primordial factor is 2 pi^2 A_s (k/k_p)^(n_s-1) k^-3.
+ */
+
+/* get_power (perturbations.py:1101-1123): P[c][i] = factor * y[c][i][idx]^2. */
+int deb_power(int n_cosmo, int nk, const double *k, const double *y, int nfield, int idx, const double *A_s,
+              const double *n_s, const double *k_p, double *P, void *stream);
+
+/* Device scratch (bytes) of the smoothing coefficients: (polyorder + 1) x
+ * window_length doubles per cosmology. */
+size_t deb_observables_scratch_bytes(int n_cosmo, int window_length, int polyorder);
+
+/* get_power_smoothed (perturbations.py:1126-1159): exp of the Savitzky-Golay
+ * filter (util.py:407-444; window_length, polyorder as there, window_length
+ * <= nk <= DEB_OBS_MAXK) of log P, with the first window_length/2 and the last
+ * (window_length+1)/2 points left unsmoothed.  The caller computes
+ * window_length from dlogk as the reference does. */
+int deb_power_smoothed(int n_cosmo, int nk, const double *k, const double *y, int nfield, int idx,
+                       const double *A_s, const double *n_s, const double *k_p, int window_length, int polyorder,
+                       void *scratch, size_t scratch_bytes, double *P_smooth, void *stream);
+
+/* power_Kaiser (perturbations.py:1162-1198): Pkmu[c][i][j] = (bias delta -
+ * mu_j^2 theta)^2 for mu device [nmu].  window_length 0: delta, theta from the
+ * fields and the primordial factor; otherwise from the smoothed spectra of
+ * fields 4 and 5 (delta = +sqrt, theta = -sqrt), as smooth_dlogk does. */
+int deb_power_kaiser(int n_cosmo, int nk, int nmu, const double *k, const double *y, int nfield,
+                     const double *A_s, const double *n_s, const double *k_p, const double *bias, const double *mu,
+                     int window_length, int polyorder, void *scratch, size_t scratch_bytes, double *Pkmu,
+                     void *stream);
+
+/* power_multipoles (perturbations.py:1202-1224): P0, P2, P4 [n_cosmo][nk]. */
+int deb_power_multipoles(int n_cosmo, int nk, const double *k, const double *y, int nfield, const double *A_s,
+                         const double *n_s, const double *k_p, const double *bias, double *P0, double *P2,
+                         double *P4, void *stream);
+
+/* get_xi_from_P (perturbations.py:1065-1098): FFTlog of Pk [n_cosmo][N] on the
+ * log-spaced k [n_cosmo][N] (N even, <= DEB_OBS_MAXK) for multipole ell;
+ * xi and r [n_cosmo][N] in increasing r. */
+int deb_xi_from_power(int n_cosmo, int N, const double *k, const double *Pk, int ell, double *xi, double *r,
+                      void *stream);
 
 #ifdef __cplusplus
 }
