@@ -159,8 +159,12 @@ __device__ __forceinline__ void bsolveT(const Mode<NS> &m, double (&x)[NS]) {
 // x <- W^{-T} x (transpose of wsolve); a call, so that the step factors stay
 // in memory rather than in registers across the reverse stage loop
 template <int NS>
-DEB_OUTLINE void wsolveT(const Mode<NS> &m, double (&x)[NS]) {
+DEB_OUTLINE void wsolveT(const Mode<NS> &m, double (&xio)[NS]) {
   const int lane = lane_id();
+  // a private copy (stores through xio could alias m for the compiler)
+  double x[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) x[s] = xio[s];
   const double kba = readlane(x[0], L_A), kb1 = readlane(x[0], L_AHP);
   if (lane == L_A || lane == L_AHP) x[0] = 0.0;
   const double r1b = m.gh * kb1;
@@ -190,6 +194,8 @@ DEB_OUTLINE void wsolveT(const Mode<NS> &m, double (&x)[NS]) {
   kab += wave_sum1(pj);
   if (lane == L_A) x[0] = m.inv_kaden * kab;
   if (lane == L_AHP) x[0] = r1b;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) xio[s] = x[s];
 }
 
 // ---------------------------------------------------------------------------
@@ -199,9 +205,13 @@ DEB_OUTLINE void wsolveT(const Mode<NS> &m, double (&x)[NS]) {
 // and the a-row are handled through the features).  T = dual gives
 // (A^T w, A'(a)^T w).
 template <int NS, class T>
-DEB_OUTLINE void tapply(Ctx &cx, const uint32_t (&rd)[NS], T a, double invtau, const double (&w)[NS],
-                                       T (&out)[NS]) {
+DEB_OUTLINE void tapply(Ctx &cx, const uint32_t (&rd)[NS], T a, double invtau, const double (&w_in)[NS],
+                        T (&out_io)[NS]) {
   const int lane = lane_id();
+  double w[NS];
+  T out[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) w[s] = w_in[s];
   Bg<T> b = background<T>(cx, a);
   T su[3];
   su[0] = tconst<T>(0.0);
@@ -231,7 +241,8 @@ DEB_OUTLINE void tapply(Ctx &cx, const uint32_t (&rd)[NS], T a, double invtau, c
   t_wave_sum<T, 3>(su);
   const double wahp = readlane(w[0], L_AHP);
 #pragma unroll
-  for (int s = 0; s < NS; ++s) out[s] = out[s] + cc[0][s] * su[0] + cc[1][s] * su[1] + cc[2][s] * su[2] + cc[3][s] * wahp;
+  for (int s = 0; s < NS; ++s)
+    out_io[s] = out[s] + cc[0][s] * su[0] + cc[1][s] * su[1] + cc[2][s] * su[2] + cc[3][s] * wahp;
 }
 
 // ---------------------------------------------------------------------------
@@ -293,10 +304,21 @@ using RowSums = RowSumsT<double>;
 
 template <int NS, bool TAPPLY>
 DEB_OUTLINE RowSums coef_pass(Ctx &cx, const uint32_t (&rd)[NS], double a, double invtau,
-                              const double (&w)[NS], const double (&Y)[NS], double (&out)[NS], double (&fv)[FT_N]) {
+                              const double (&w_in)[NS], const double (&Y_in)[NS], double (&out)[NS],
+                              double (&fv_out)[FT_N]) {
   const int lane = lane_id();
+  double w[NS], Y[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    w[s] = w_in[s];
+    Y[s] = Y_in[s];
+  }
   const Bg<double> b = background<double>(cx, a);
+  double fv[FT_N];
   features_of<double>(cx, a, b, fv);
+#pragma unroll
+  for (int m = 0; m < FT_N; ++m) fv_out[m] = fv[m];
+  double o[NS];
   double part[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   double cc[4][NS];
 #pragma unroll
@@ -310,12 +332,12 @@ DEB_OUTLINE RowSums coef_pass(Ctx &cx, const uint32_t (&rd)[NS], double a, doubl
     part[5] += r.ueta * w[s];
     part[6] += r.upi * w[s];
     if constexpr (TAPPLY) {
-      out[s] = r.diag * w[s] + from_next(r.sub * w[s]) + from_prev(r.sup * w[s]);
+      o[s] = r.diag * w[s] + from_next(r.sub * w[s]) + from_prev(r.sup * w[s]);
       if (s == 0) {
         const double lw = r.leaf * w[0];
         const double l_dg = readlane(lw, L_DG), l_tg = readlane(lw, L_TG);
-        if (lane == L_TG) out[0] += l_dg;
-        if (lane == L_DG) out[0] += l_tg;
+        if (lane == L_TG) o[0] += l_dg;
+        if (lane == L_DG) o[0] += l_tg;
       }
       cc[0][s] = r.ch;
       cc[1][s] = r.ceta;
@@ -328,7 +350,7 @@ DEB_OUTLINE RowSums coef_pass(Ctx &cx, const uint32_t (&rd)[NS], double a, doubl
   if constexpr (TAPPLY) {
     const double wahp = readlane(w[0], L_AHP);
 #pragma unroll
-    for (int s = 0; s < NS; ++s) out[s] += cc[0][s] * rs.sh + cc[1][s] * rs.se + cc[2][s] * rs.sp + cc[3][s] * wahp;
+    for (int s = 0; s < NS; ++s) out[s] = o[s] + cc[0][s] * rs.sh + cc[1][s] * rs.se + cc[2][s] * rs.sp + cc[3][s] * wahp;
   }
   return rs;
 }
@@ -342,6 +364,7 @@ __device__ __forceinline__ void row_feature_adj(uint32_t d, const Ctx &cx, const
                                                 const T (&vq)[K], const T (&mul)[K][11], T (&fb)[K][FT_N],
                                                 T (&vb)[K]) {
   const double k = cx.k, k2 = k * k;
+  const double ik2 = 1.0 / k2, hk2 = 0.5 * ik2;  // wave-uniform
   const uint32_t t = rd_type(d);
   const double l = (double)rd_l(d);
   const int qi = (int)rd_q(d);
@@ -372,7 +395,7 @@ __device__ __forceinline__ void row_feature_adj(uint32_t d, const Ctx &cx, const
 #pragma unroll
       for (int q = 0; q < K; ++q) {
         fb[q][FT_AH] += -mul[q][C_DIAG];
-        fb[q][FT_RC] += mul[q][C_CETA] * (0.5 / k2);
+        fb[q][FT_RC] += mul[q][C_CETA] * hk2;
       }
       break;
     case R_TB:
@@ -382,7 +405,7 @@ __device__ __forceinline__ void row_feature_adj(uint32_t d, const Ctx &cx, const
         fb[q][FT_AH] += -mul[q][C_DIAG];
         fb[q][FT_PB43] += (mul[q][C_SUP] - mul[q][C_DIAG]) * fv[q][FT_OPAC];
         fb[q][FT_OPAC] += (mul[q][C_SUP] - mul[q][C_DIAG]) * fv[q][FT_PB43];
-        fb[q][FT_RB] += mul[q][C_CETA] * (0.5 / k2);
+        fb[q][FT_RB] += mul[q][C_CETA] * hk2;
       }
       break;
     case R_DG:
@@ -397,7 +420,7 @@ __device__ __forceinline__ void row_feature_adj(uint32_t d, const Ctx &cx, const
 #pragma unroll
       for (int q = 0; q < K; ++q) {
         fb[q][FT_OPAC] += mul[q][C_SUB] - mul[q][C_DIAG];
-        fb[q][FT_RG] += mul[q][C_CETA] * (4.0 / 3.0) * (0.5 / k2);
+        fb[q][FT_RG] += mul[q][C_CETA] * (4.0 / 3.0) * hk2;
       }
       break;
     case R_F2:
@@ -438,7 +461,7 @@ __device__ __forceinline__ void row_feature_adj(uint32_t d, const Ctx &cx, const
     case R_TR:
 #pragma unroll
       for (int q = 0; q < K; ++q) {
-        fb[q][FT_RR] += mul[q][C_CETA] * (4.0 / 3.0) * (0.5 / k2);
+        fb[q][FT_RR] += mul[q][C_CETA] * (4.0 / 3.0) * hk2;
       }
       break;
     case R_Q0: {
@@ -446,7 +469,7 @@ __device__ __forceinline__ void row_feature_adj(uint32_t d, const Ctx &cx, const
       for (int q = 0; q < K; ++q) {
         const double wq = cx.w[qi];
         const T RN = fv[q][FT_RN];
-        const T iv = 1.0 / vq[q];
+        const T iv = rcp_t<T>(vq[q]);
         vb[q] += -k * mul[q][C_SUP];
         // ch = RN wq / v * iaH
         fb[q][FT_RN] += mul[q][C_CH] * wq * iv * fv[q][FT_IAH];
@@ -462,8 +485,8 @@ __device__ __forceinline__ void row_feature_adj(uint32_t d, const Ctx &cx, const
     case R_Q1:
 #pragma unroll
       for (int q = 0; q < K; ++q) {
-        vb[q] += k * (mul[q][C_SUB] / 3.0 - 2.0 * mul[q][C_SUP] / 3.0);
-        fb[q][FT_RN] += mul[q][C_CETA] * k * cx.w[qi] * (0.5 / k2);
+        vb[q] += k * (mul[q][C_SUB] * (1.0 / 3.0) - mul[q][C_SUP] * (2.0 / 3.0));
+        fb[q][FT_RN] += mul[q][C_CETA] * k * cx.w[qi] * hk2;
       }
       break;
     case R_Q2:
@@ -472,12 +495,14 @@ __device__ __forceinline__ void row_feature_adj(uint32_t d, const Ctx &cx, const
         vb[q] += k * (0.4 * mul[q][C_SUB] - 0.6 * mul[q][C_SUP]);
       }
       break;
-    case R_QL:
+    case R_QL: {
+      const double il = rcp_nr(2.0 * l + 1.0);
 #pragma unroll
       for (int q = 0; q < K; ++q) {
-        vb[q] += k * (mul[q][C_SUB] * (l / (2.0 * l + 1.0)) - mul[q][C_SUP] * ((l + 1.0) / (2.0 * l + 1.0)));
+        vb[q] += k * (mul[q][C_SUB] * (l * il) - mul[q][C_SUP] * ((l + 1.0) * il));
       }
       break;
+    }
     case R_QLMAX:
 #pragma unroll
       for (int q = 0; q < K; ++q) {
@@ -489,7 +514,7 @@ __device__ __forceinline__ void row_feature_adj(uint32_t d, const Ctx &cx, const
       for (int q = 0; q < K; ++q) {
         if (!cx.lambda) {
           const T w = fv[q][FT_W], ca2 = fv[q][FT_CA2], cs2Q = fv[q][FT_CS2Q], RQ = fv[q][FT_RQ];
-          const T opw = 1.0 + w, g = fv[q][FT_AH] * fv[q][FT_AH] / k2;
+          const T opw = 1.0 + w, g = fv[q][FT_AH] * fv[q][FT_AH] * ik2;
           // diag = -3 (cs2Q - w) aH
           fb[q][FT_CS2Q] += -3.0 * fv[q][FT_AH] * mul[q][C_DIAG];
           fb[q][FT_W] += 3.0 * fv[q][FT_AH] * mul[q][C_DIAG];
@@ -498,7 +523,7 @@ __device__ __forceinline__ void row_feature_adj(uint32_t d, const Ctx &cx, const
           fb[q][FT_W] += mul[q][C_SUP] * (-1.0 - 9.0 * (cs2Q - ca2) * g);
           fb[q][FT_CS2Q] += -9.0 * opw * g * mul[q][C_SUP];
           fb[q][FT_CA2] += 9.0 * opw * g * mul[q][C_SUP];
-          fb[q][FT_AH] += -18.0 * opw * (cs2Q - ca2) * fv[q][FT_AH] / k2 * mul[q][C_SUP];
+          fb[q][FT_AH] += -18.0 * opw * (cs2Q - ca2) * fv[q][FT_AH] * ik2 * mul[q][C_SUP];
           // uh = -0.5 opw
           fb[q][FT_W] += -0.5 * mul[q][C_UH];
           // ch = RQ iaH
@@ -518,16 +543,17 @@ __device__ __forceinline__ void row_feature_adj(uint32_t d, const Ctx &cx, const
           const T w = fv[q][FT_W], ca2 = fv[q][FT_CA2], cs2Q = fv[q][FT_CS2Q], RQ = fv[q][FT_RQ];
           const T opw = 1.0 + w;
           // sub = cs2Q / opw k2
-          fb[q][FT_CS2Q] += mul[q][C_SUB] * k2 / opw;
-          fb[q][FT_W] += -mul[q][C_SUB] * cs2Q * k2 / (opw * opw);
+          const T iopw = rcp_t<T>(opw);
+          fb[q][FT_CS2Q] += mul[q][C_SUB] * k2 * iopw;
+          fb[q][FT_W] += -mul[q][C_SUB] * cs2Q * k2 * (iopw * iopw);
           // diag = -(1 - 3 cs2Q) aH
           fb[q][FT_CS2Q] += 3.0 * fv[q][FT_AH] * mul[q][C_DIAG];
           fb[q][FT_AH] += -(1.0 - 3.0 * cs2Q) * mul[q][C_DIAG];
           // ceta = opw RQ 0.5 / k2
-          fb[q][FT_W] += mul[q][C_CETA] * RQ * (0.5 / k2);
-          fb[q][FT_RQ] += mul[q][C_CETA] * opw * (0.5 / k2);
+          fb[q][FT_W] += mul[q][C_CETA] * RQ * hk2;
+          fb[q][FT_RQ] += mul[q][C_CETA] * opw * hk2;
           // c1 = -a 9 (cs2Q - ca2) aH opw RQ / k2
-          const double c9 = 9.0 / k2;
+          const double c9 = 9.0 * ik2;
           fb[q][FT_A] += -mul[q][C_C1] * c9 * (cs2Q - ca2) * fv[q][FT_AH] * opw * RQ;
           fb[q][FT_CS2Q] += -mul[q][C_C1] * fv[q][FT_A] * c9 * fv[q][FT_AH] * opw * RQ;
           fb[q][FT_CA2] += mul[q][C_C1] * fv[q][FT_A] * c9 * fv[q][FT_AH] * opw * RQ;
@@ -586,14 +612,25 @@ __device__ __forceinline__ void row_muls(const double (&Y)[NS], const double (&w
 // fb[q] += partials of sum_rows sum_fields mul[q][row][field] coef_field with
 // respect to the features fv[q] (K contractions in one pass over the rows)
 template <int NS, int K, class T = double>
-DEB_OUTLINE void feature_partials_m(const Ctx &cx, const uint32_t (&rd)[NS], const T (&fv)[K][FT_N],
+DEB_OUTLINE void feature_partials_m(const Ctx &cx, const uint32_t (&rd)[NS], const T (&fv_in)[K][FT_N],
                                     const T (&mul)[K][NS][11], T (&fb)[K][FT_N]) {
   const int lane = lane_id();
+  // Private copies: fb may alias fv and mul as far as the compiler knows, so
+  // accumulating through the references would reload every operand after each
+  // store (memory traffic in the caller's frame); locals stay in registers.
+  T fv[K][FT_N], acc[K][FT_N];
+#pragma unroll
+  for (int q = 0; q < K; ++q)
+#pragma unroll
+    for (int m = 0; m < FT_N; ++m) {
+      fv[q][m] = fv_in[q][m];
+      acc[q][m] = tconst<T>(0.0);
+    }
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
     const uint32_t ty = rd_type(rd[s]);
     const bool qrow = ty >= R_Q0 && ty <= R_QLMAX;
-    const double iq = qrow ? 1.0 / cx.q[rd_q(rd[s])] : 0.0;
+    const double iq = qrow ? rcp_nr(cx.q[rd_q(rd[s])]) : 0.0;
     T ms[K][11], vq[K], xq[K], vb[K];
 #pragma unroll
     for (int q = 0; q < K; ++q) {
@@ -601,26 +638,30 @@ DEB_OUTLINE void feature_partials_m(const Ctx &cx, const uint32_t (&rd)[NS], con
       for (int f = 0; f < 11; ++f) ms[q][f] = mul[q][s][f];
       // v_q(x), x = a amnu / q
       xq[q] = fv[q][FT_A] * fv[q][FT_AMNU] * iq;
-      vq[q] = 1.0 / dsqrt(1.0 + xq[q] * xq[q]);
+      vq[q] = rsqrt_t<T>(1.0 + xq[q] * xq[q]);
       vb[q] = tconst<T>(0.0);
     }
-    row_feature_adj<K, T>(rd[s], cx, fv, vq, ms, fb, vb);
+    row_feature_adj<K, T>(rd[s], cx, fv, vq, ms, acc, vb);
     if (qrow) {
 #pragma unroll
       for (int q = 0; q < K; ++q) {
         const T dvdx = -xq[q] * vq[q] * vq[q] * vq[q];
-        fb[q][FT_A] += vb[q] * dvdx * fv[q][FT_AMNU] * iq;
-        fb[q][FT_AMNU] += vb[q] * dvdx * fv[q][FT_A] * iq;
+        acc[q][FT_A] += vb[q] * dvdx * fv[q][FT_AMNU] * iq;
+        acc[q][FT_AMNU] += vb[q] * dvdx * fv[q][FT_A] * iq;
       }
     }
   }
   if (lane == L_A) {  // f_a = aH a, weight w_a (row_muls)
 #pragma unroll
     for (int q = 0; q < K; ++q) {
-      fb[q][FT_AH] += mul[q][0][C_LEAF] * fv[q][FT_A];
-      fb[q][FT_A] += mul[q][0][C_LEAF] * fv[q][FT_AH];
+      acc[q][FT_AH] += mul[q][0][C_LEAF] * fv[q][FT_A];
+      acc[q][FT_A] += mul[q][0][C_LEAF] * fv[q][FT_AH];
     }
   }
+#pragma unroll
+  for (int q = 0; q < K; ++q)
+#pragma unroll
+    for (int m = 0; m < FT_N; ++m) fb[q][m] += acc[q][m];
 }
 
 // One tangent direction per lane: lane & 7 == 0 carries d/da (component 0 of
@@ -656,7 +697,12 @@ __device__ __forceinline__ double lane_dot(const TL (&F)[FT_N], const double (&f
 }
 
 // feature values at a (plain doubles)
-DEB_OUTLINE void feature_values(Ctx &cx, double a, double (&fv)[FT_N]) { features_t<double>(cx, a, fv); }
+DEB_OUTLINE void feature_values(Ctx &cx, double a, double (&fv)[FT_N]) {
+  double f[FT_N];
+  features_t<double>(cx, a, f);
+#pragma unroll
+  for (int m = 0; m < FT_N; ++m) fv[m] = f[m];
+}
 
 // the lane's direction of sum_m F_m(a) fbs_m (fbs wave-uniform): d/da on
 // lanes with lane & 7 == 0, d/dtheta_{c-1} on lanes with lane & 7 == c
@@ -668,13 +714,24 @@ DEB_OUTLINE double lane_contract(const Ctx &cx, double a, const double (&fbs)[FT
 }
 
 template <class T>
-DEB_OUTLINE void features_call(Ctx &cx, T a, T (&F)[FT_N]) { features_t<T>(cx, a, F); }
+DEB_OUTLINE void features_call(Ctx &cx, T a, T (&F)[FT_N]) {
+  T f[FT_N];
+  features_t<T>(cx, a, f);
+#pragma unroll
+  for (int m = 0; m < FT_N; ++m) F[m] = f[m];
+}
 
 // the functional sums of sum_rows w f_row(a, Y) in T (for T = dual along a:
 // their values and a-derivatives)
 template <int NS, class T>
-DEB_OUTLINE void row_sums_t(Ctx &cx, const uint32_t (&rd)[NS], T a, double invtau, const double (&w)[NS],
-                            const double (&Y)[NS], RowSumsT<T> &rs) {
+DEB_OUTLINE void row_sums_t(Ctx &cx, const uint32_t (&rd)[NS], T a, double invtau, const double (&w_in)[NS],
+                            const double (&Y_in)[NS], RowSumsT<T> &rs) {
+  double w[NS], Y[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    w[s] = w_in[s];
+    Y[s] = Y_in[s];
+  }
   const Bg<T> b = background<T>(cx, a);
   T part[7];
 #pragma unroll
@@ -1173,10 +1230,13 @@ __device__ __forceinline__ void output_vjp(const Params<NS> &P, Ctx &cx, const A
 
 // one (cosmology, k) item of the gradient kernel.  ring: this wave's ring
 // (ring_doubles(NS, C) doubles of device memory, owned for the item's life);
-// KBl, ls, lg: per-wave LDS scratch.
+// KBl, ls, lg: per-wave LDS scratch; cx: the wave's context record, in LDS in
+// the kernel (its fields are wave-uniform, so one copy serves the wave, and the
+// row passes read it there instead of from a per-lane stack copy; every lane
+// stores the same values into it)
 template <int NS>
 __device__ __forceinline__ void adjoint_mode(const Params<NS> &P, const AdjArgs &A, int ci, int ki, double *ring,
-                                             double *KBl, double *ls, double *lg, lds_double *pcr) {
+                                             double *KBl, double *ls, double *lg, lds_double *pcr, Ctx &cx) {
   const int lane = lane_id();
   const long long item = (long long)ci * P.nk + ki;
   double *gout = A.grad + item * A.ndir;
@@ -1184,7 +1244,6 @@ __device__ __forceinline__ void adjoint_mode(const Params<NS> &P, const AdjArgs 
     if (lane < A.ndir) gout[lane] = __builtin_nan("");
     return;
   }
-  Ctx cx;
   const double k = P.kmodes[item];
   make_ctx<NS>(cx, P, ci, k);
   cx.ndir = A.ndir;

@@ -32,6 +32,7 @@ __global__ void __launch_bounds__(WAVE) deb_adjoint_kernel(const Params<NS> P, c
   // [NS][64] blocks stage the output conversion's reverse (output_vjp)
   __shared__ double KBl[7 * NS * WAVE];
   __shared__ double pcr[pcr_lds_doubles<NS>()];
+  __shared__ Ctx cxs;
   const int item = blockIdx.x;
   if (item >= P.n_cosmo * P.nk) return;
   const int ki = P.nk - 1 - item / P.n_cosmo;
@@ -54,7 +55,7 @@ __global__ void __launch_bounds__(WAVE) deb_adjoint_kernel(const Params<NS> P, c
     return;
   }
   adjoint_mode<NS>(P, A, ci, ki, A.rings + (long long)ring * ring_doubles(NS, A.tape.ckpt), KBl, KBl,
-                   KBl + NS * WAVE, (lds_double *)pcr);
+                   KBl + NS * WAVE, (lds_double *)pcr, cxs);
   __threadfence();
   if (lane == 0) atomicExch(&A.ring_busy[ring], 0);
 }

@@ -103,8 +103,9 @@ __device__ __forceinline__ dual operator-(dual a, dual b) { return {a.v - b.v, a
 __device__ __forceinline__ dual operator-(dual a) { return {-a.v, -a.d}; }
 __device__ __forceinline__ dual operator*(dual a, dual b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
 __device__ __forceinline__ dual operator/(dual a, dual b) {
-  double q = a.v / b.v;
-  return {q, (a.d - q * b.d) / b.v};
+  const double ib = 1.0 / b.v;
+  const double q = a.v * ib;
+  return {q, (a.d - q * b.d) * ib};
 }
 __device__ __forceinline__ dual operator+(dual a, double b) { return {a.v + b, a.d}; }
 __device__ __forceinline__ dual operator+(double b, dual a) { return {a.v + b, a.d}; }
@@ -114,8 +115,9 @@ __device__ __forceinline__ dual operator*(dual a, double b) { return {a.v * b, a
 __device__ __forceinline__ dual operator*(double b, dual a) { return {a.v * b, a.d * b}; }
 __device__ __forceinline__ dual operator/(dual a, double b) { return {a.v / b, a.d / b}; }
 __device__ __forceinline__ dual operator/(double b, dual a) {
-  double q = b / a.v;
-  return {q, -q * a.d / a.v};
+  const double ia = 1.0 / a.v;
+  const double q = b * ia;
+  return {q, -q * a.d * ia};
 }
 __device__ __forceinline__ dual dexp(dual a) {
   double e = exp(a.v);
@@ -134,6 +136,26 @@ __device__ __forceinline__ double dexp(double a) { return exp(a); }
 __device__ __forceinline__ double dlog(double a) { return log(a); }
 __device__ __forceinline__ double dsqrt(double a) { return sqrt(a); }
 __device__ __forceinline__ double dpow(double a, double p) { return pow(a, p); }
+
+// 1/x and 1/sqrt(x) for positive normal x: the hardware estimate refined by
+// two Newton steps, within about one ulp (the IEEE division and square root
+// add scaling and a final correction step that these per-lane row constants
+// do not need)
+__device__ __forceinline__ double rcp_nr(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+__device__ __forceinline__ double rsq_nr(double x) {
+  double r = __builtin_amdgcn_rsq(x);
+  const double h = 0.5 * x;
+  double e = fma(-h * r, r, 0.5);
+  r = fma(r, e, r);
+  e = fma(-h * r, r, 0.5);
+  return fma(r, e, r);
+}
 __device__ __forceinline__ double vval(double a) { return a; }
 __device__ __forceinline__ double vval(dual a) { return a.v; }
 __device__ __forceinline__ double dval(double) { return 0.0; }

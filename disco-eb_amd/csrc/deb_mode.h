@@ -82,22 +82,34 @@ __device__ __forceinline__ T spl_at(Ctx &cx, int s, T xv) {
 
 // rho_Q(a) / rho_Q(1) = a^{-3(1+w0+wa)} exp(3 (a-1) wa); for dn types the
 // exponent's own tangent enters through exp((e - e0) log a)
+// (one exponential of -3 (1 + w0 + wa) log a + 3 (a - 1) wa; the exponent
+// carries the parameters' tangents for dn types)
 template <class T>
 __device__ __forceinline__ T rho_de(T a, T loga, ParT<T> w0, ParT<T> wa) {
-  const double e0 = vval(-3.0 * (1.0 + w0 + wa));
-  T r = dpow(a, e0) * dexp(3.0 * (a - 1.0) * wa);
-  if constexpr (is_dn<T>::value) r = r * dexp((-3.0 * (1.0 + w0 + wa) - e0) * loga);
-  return r;
+  return dexp((-3.0 * (1.0 + w0 + wa)) * loga + 3.0 * (a - 1.0) * wa);
 }
 
+// 1/sqrt(x): refined hardware estimate for plain doubles
+template <class T>
+__device__ __forceinline__ T rsqrt_t(T x) {
+  if constexpr (std::is_same<T, double>::value) return rsq_nr(x);
+  else return 1.0 / dsqrt(x);
+}
+
+// 1/x: refined hardware estimate for plain doubles
+template <class T>
+__device__ __forceinline__ T rcp_t(T x) {
+  if constexpr (std::is_same<T, double>::value) return rcp_nr(x);
+  else return 1.0 / x;
+}
+
+// background quantities at a, with the reciprocals 1/a, 1/a^2 and 1/(aH) that
+// the row coefficients use (taken once per evaluation, not per row)
 template <typename T>
 struct Bg {
   T a, aH, opac, cs2, pb43, rhoQ, w, ca2;
+  T ia, ia2, iaH;
 };
-
-// massive-nu velocity v = 1/sqrt(1 + (a m_nu / q)^2) of momentum bin qi
-template <typename T>
-__device__ __forceinline__ T nu_v(const Ctx &cx, T a, int qi);
 
 
 
@@ -127,27 +139,25 @@ __device__ __forceinline__ Bg<T> background(Ctx &cx, T a) {
     xe = r3[1];
     lrhonu = r3[2];
   }
-  b.cs2 = cs2a / a;
+  b.ia = 1.0 / a;
+  b.ia2 = b.ia * b.ia;
+  b.cs2 = cs2a * b.ia;
   T rhonu = dexp(lrhonu);
-  b.pb43 = (4.0 / 3.0) * (grhog / (grhom * Omegab)) / a;
+  b.pb43 = (4.0 / 3.0) * (grhog / (grhom * Omegab)) * b.ia;
   b.w = w0 + wa * (1.0 - a);
   b.rhoQ = rho_de<T>(a, loga, w0, wa);
   T a2 = a * a;
-  T grho = grhom * Omegam / a + (grhog + grhor * (Neff + Nmnu * rhonu)) / a2 + grhom * OmegaDE * b.rhoQ * a2 +
+  T grho = grhom * Omegam * b.ia + (grhog + grhor * (Neff + Nmnu * rhonu)) * b.ia2 + grhom * OmegaDE * b.rhoQ * a2 +
            grhom * Omegak;
-  b.aH = dsqrt(grho / 3.0);
+  b.aH = dsqrt(grho * (1.0 / 3.0));
+  b.iaH = 1.0 / b.aH;
   T wprime = -wa * b.aH * a;
-  b.ca2 = b.w - wprime / 3.0 / ((1.0 + b.w) + 1e-6) / b.aH;
+  b.ca2 = b.w - wprime * (1.0 / 3.0) / ((1.0 + b.w) + 1e-6) * b.iaH;
   const P akthom = 2.3038921003709498e-9 * (1.0 - YHe) * Omegab * H0 * H0;
-  b.opac = xe * akthom / a2;
+  b.opac = xe * akthom * b.ia2;
   return b;
 }
 
-template <typename T>
-__device__ __forceinline__ T nu_v(const Ctx &cx, T a, int qi) {
-  T aq = a * (cv<T>(cx, F_AMNU) / cx.q[qi]);
-  return 1.0 / dsqrt(1.0 + aq * aq);
-}
 
 template <typename T>
 struct Coef {
@@ -161,6 +171,12 @@ struct Coef {
 // coefficient fields of a row
 enum : int { C_SUB = 0, C_DIAG, C_SUP, C_UH, C_UETA, C_UPI, C_CH, C_CETA, C_CPI, C_C1, C_LEAF };
 
+// The coefficient fields of one row (model_synchronous, perturbations.py:
+// 84-371), each passed to sink(field, value).  Every case sets each of its
+// fields once.  The formulas use the evaluation's reciprocals (b.ia, b.ia2,
+// b.iaH) and per-mode constants, so a case body holds multiplications only:
+// the divergent switch runs every case present in a slot, and a division per
+// case multiplied its cost.
 template <typename T, class Sink>
 __device__ __forceinline__ void row_visit(uint32_t d, const Bg<T> &b, const Ctx &cx, double invtau, Sink sink) {
   using P = ParT<T>;
@@ -168,60 +184,71 @@ __device__ __forceinline__ void row_visit(uint32_t d, const Bg<T> &b, const Ctx 
   const P Omegam = cv<T>(cx, F_OMEGAM), Omegab = cv<T>(cx, F_OMEGAB), OmegaDE = cv<T>(cx, F_OMEGADE);
   const P Neff = cv<T>(cx, F_NEFF), Nmnu = cv<T>(cx, F_NMNU), cs2DE = cv<T>(cx, F_CS2DE);
   const double k = cx.k, k2 = k * k;
+  const double ik2 = 1.0 / k2, hk2 = 0.5 * ik2;  // wave-uniform
   const uint32_t t = rd_type(d);
   const double l = (double)rd_l(d);
   const int qi = (int)rd_q(d);
   const T a = b.a;
-  const T a2 = a * a;
+  const T ia = b.ia, ia2 = b.ia2, iaH = b.iaH;
   const P Omegac = Omegam - Omegab;
-  const T inv_aH = 1.0 / b.aH;
   // multipole couplings l/(2l+1) and (l+1)/(2l+1), and the massive-nu velocity
   // of the row's momentum bin: evaluated once here rather than in each of the
   // (divergent) row cases that use them
-  const double il = 1.0 / (2.0 * l + 1.0);
+  const double il = rcp_nr(2.0 * l + 1.0);
   const double lsub = l * il, lsup = (l + 1.0) * il;
   const bool qrow = t >= R_Q0 && t <= R_QLMAX;
-  T vq = tconst<T>(1.0);
-  if (qrow) vq = nu_v<T>(cx, a, qi);
+  T vq = tconst<T>(1.0), ivq = tconst<T>(1.0);  // v_q and 1/v_q = sqrt(1 + x^2)
+  if (qrow) {
+    const T aq = a * (cv<T>(cx, F_AMNU) * rcp_nr(cx.q[qi]));
+    const T s2 = 1.0 + aq * aq;
+    vq = rsqrt_t<T>(s2);
+    ivq = s2 * vq;
+  }
   switch (t) {
     case R_ETA:
       sink(std::integral_constant<int, C_UETA>{}, 1.0);
-      sink(std::integral_constant<int, C_CH>{}, 2.0 * k2 * inv_aH);
+      sink(std::integral_constant<int, C_CH>{}, (2.0 * k2) * iaH);
       break;
-    case R_DC:
+    case R_DC: {
+      const P rc = grhom * Omegac;
       sink(std::integral_constant<int, C_SUP>{}, -1.0);
       sink(std::integral_constant<int, C_UH>{}, -0.5);
-      sink(std::integral_constant<int, C_CH>{}, grhom * Omegac / a * inv_aH);
-      sink(std::integral_constant<int, C_C1>{}, -a * (grhom * Omegac / a));
+      sink(std::integral_constant<int, C_CH>{}, rc * ia * iaH);
+      sink(std::integral_constant<int, C_C1>{}, -rc);
       break;
+    }
     case R_TC:
       sink(std::integral_constant<int, C_DIAG>{}, -b.aH);
-      sink(std::integral_constant<int, C_CETA>{}, grhom * Omegac / a * (0.5 / k2));
+      sink(std::integral_constant<int, C_CETA>{}, grhom * Omegac * ia * hk2);
       break;
-    case R_DB:
+    case R_DB: {
+      const P rb = grhom * Omegab;
       sink(std::integral_constant<int, C_SUP>{}, -1.0);
       sink(std::integral_constant<int, C_UH>{}, -0.5);
-      sink(std::integral_constant<int, C_CH>{}, grhom * Omegab / a * inv_aH);
-      sink(std::integral_constant<int, C_C1>{}, -a * (grhom * Omegab / a));
+      sink(std::integral_constant<int, C_CH>{}, rb * ia * iaH);
+      sink(std::integral_constant<int, C_C1>{}, -rb);
       break;
-    case R_TB:
+    }
+    case R_TB: {
+      const T po = b.pb43 * b.opac;
       sink(std::integral_constant<int, C_SUB>{}, k2 * b.cs2);
-      sink(std::integral_constant<int, C_DIAG>{}, -b.aH - b.pb43 * b.opac);
-      sink(std::integral_constant<int, C_SUP>{}, b.pb43 * b.opac);
-      sink(std::integral_constant<int, C_CETA>{}, grhom * Omegab / a * (0.5 / k2));
+      sink(std::integral_constant<int, C_DIAG>{}, -b.aH - po);
+      sink(std::integral_constant<int, C_SUP>{}, po);
+      sink(std::integral_constant<int, C_CETA>{}, grhom * Omegab * ia * hk2);
       break;
+    }
     case R_DG:
       sink(std::integral_constant<int, C_LEAF>{}, -4.0 / 3.0);
       sink(std::integral_constant<int, C_UH>{}, -2.0 / 3.0);
-      sink(std::integral_constant<int, C_CH>{}, grhog / a2 * inv_aH);
-      sink(std::integral_constant<int, C_C1>{}, -a * (grhog / a2 + 3.0 * grhog / a2 / 3.0));
+      sink(std::integral_constant<int, C_CH>{}, grhog * ia2 * iaH);
+      sink(std::integral_constant<int, C_C1>{}, (-2.0 * grhog) * ia);
       break;
     case R_TG:
       sink(std::integral_constant<int, C_SUB>{}, b.opac);
       sink(std::integral_constant<int, C_DIAG>{}, -b.opac);
       sink(std::integral_constant<int, C_SUP>{}, -0.5 * k2);
       sink(std::integral_constant<int, C_LEAF>{}, 0.25 * k2);
-      sink(std::integral_constant<int, C_CETA>{}, (4.0 / 3.0) * grhog / a2 * (0.5 / k2));
+      sink(std::integral_constant<int, C_CETA>{}, (4.0 / 3.0) * grhog * ia2 * hk2);
       break;
     case R_F2:
       sink(std::integral_constant<int, C_SUB>{}, 8.0 / 15.0);
@@ -253,9 +280,9 @@ __device__ __forceinline__ void row_visit(uint32_t d, const Bg<T> &b, const Ctx 
       sink(std::integral_constant<int, C_SUP>{}, -k * lsup);
       break;
     case R_G2:
-      sink(std::integral_constant<int, C_SUB>{}, k * 2.0 / 5.0);
+      sink(std::integral_constant<int, C_SUB>{}, 0.4 * k);
       sink(std::integral_constant<int, C_DIAG>{}, -b.opac);
-      sink(std::integral_constant<int, C_SUP>{}, -k * 3.0 / 5.0);
+      sink(std::integral_constant<int, C_SUP>{}, -0.6 * k);
       sink(std::integral_constant<int, C_UPI>{}, 0.1 * b.opac);
       sink(std::integral_constant<int, C_CPI>{}, 1.0);
       break;
@@ -263,16 +290,18 @@ __device__ __forceinline__ void row_visit(uint32_t d, const Bg<T> &b, const Ctx 
       sink(std::integral_constant<int, C_SUB>{}, k);
       sink(std::integral_constant<int, C_DIAG>{}, -(l + 1.0) * invtau - b.opac);
       break;
-    case R_DR:
+    case R_DR: {
+      const P rr = grhor * Neff;
       sink(std::integral_constant<int, C_SUP>{}, -4.0 / 3.0);
       sink(std::integral_constant<int, C_UH>{}, -2.0 / 3.0);
-      sink(std::integral_constant<int, C_CH>{}, grhor * Neff / a2 * inv_aH);
-      sink(std::integral_constant<int, C_C1>{}, -a * (grhor * Neff / a2 + grhor * Neff / a2));
+      sink(std::integral_constant<int, C_CH>{}, rr * ia2 * iaH);
+      sink(std::integral_constant<int, C_C1>{}, (-2.0 * rr) * ia);
       break;
+    }
     case R_TR:
       sink(std::integral_constant<int, C_SUB>{}, 0.25 * k2);
       sink(std::integral_constant<int, C_SUP>{}, -0.5 * k2);
-      sink(std::integral_constant<int, C_CETA>{}, (4.0 / 3.0) * Neff * grhor / a2 * (0.5 / k2));
+      sink(std::integral_constant<int, C_CETA>{}, (4.0 / 3.0) * Neff * grhor * ia2 * hk2);
       break;
     case R_FR2:
       sink(std::integral_constant<int, C_SUB>{}, 8.0 / 15.0);
@@ -289,30 +318,30 @@ __device__ __forceinline__ void row_visit(uint32_t d, const Bg<T> &b, const Ctx 
       sink(std::integral_constant<int, C_DIAG>{}, -(l + 1.0) * invtau);
       break;
     case R_Q0: {
-      T kv = k * vq;
-      sink(std::integral_constant<int, C_SUP>{}, -kv);
-      sink(std::integral_constant<int, C_UH>{}, cx.dlnf[qi] / 6.0);
-      sink(std::integral_constant<int, C_CH>{}, grhor * Nmnu * cx.w[qi] / vq / a2 * inv_aH);
-      sink(std::integral_constant<int, C_C1>{}, -a * (grhor * Nmnu * cx.w[qi] / vq / a2 + grhor * Nmnu * cx.w[qi] * vq / a2));
+      const P rn = grhor * Nmnu * cx.w[qi];
+      sink(std::integral_constant<int, C_SUP>{}, -k * vq);
+      sink(std::integral_constant<int, C_UH>{}, cx.dlnf[qi] * (1.0 / 6.0));
+      sink(std::integral_constant<int, C_CH>{}, rn * ivq * ia2 * iaH);
+      sink(std::integral_constant<int, C_C1>{}, -(rn * (ivq + vq)) * ia);
       break;
     }
     case R_Q1: {
-      T kv = k * vq;
-      sink(std::integral_constant<int, C_SUB>{}, kv / 3.0);
-      sink(std::integral_constant<int, C_SUP>{}, -2.0 * kv / 3.0);
-      sink(std::integral_constant<int, C_CETA>{}, Nmnu * grhor * k * cx.w[qi] / a2 * (0.5 / k2));
+      const T kv = k * vq;
+      sink(std::integral_constant<int, C_SUB>{}, kv * (1.0 / 3.0));
+      sink(std::integral_constant<int, C_SUP>{}, kv * (-2.0 / 3.0));
+      sink(std::integral_constant<int, C_CETA>{}, Nmnu * grhor * k * cx.w[qi] * ia2 * hk2);
       break;
     }
     case R_Q2: {
-      T kv = k * vq;
-      sink(std::integral_constant<int, C_SUB>{}, 2.0 * kv / 5.0);
-      sink(std::integral_constant<int, C_SUP>{}, -3.0 * kv / 5.0);
-      sink(std::integral_constant<int, C_UH>{}, -cx.dlnf[qi] / 15.0);
+      const T kv = k * vq;
+      sink(std::integral_constant<int, C_SUB>{}, 0.4 * kv);
+      sink(std::integral_constant<int, C_SUP>{}, -0.6 * kv);
+      sink(std::integral_constant<int, C_UH>{}, cx.dlnf[qi] * (-1.0 / 15.0));
       sink(std::integral_constant<int, C_UETA>{}, -0.4 * cx.dlnf[qi]);
       break;
     }
     case R_QL: {
-      T kv = k * vq;
+      const T kv = k * vq;
       sink(std::integral_constant<int, C_SUB>{}, kv * lsub);
       sink(std::integral_constant<int, C_SUP>{}, -kv * lsup);
       break;
@@ -327,10 +356,10 @@ __device__ __forceinline__ void row_visit(uint32_t d, const Bg<T> &b, const Ctx 
         const P cs2Q = cs2DE;
         T opw = 1.0 + b.w;
         sink(std::integral_constant<int, C_DIAG>{}, -3.0 * (cs2Q - b.w) * b.aH);
-        sink(std::integral_constant<int, C_SUP>{}, -opw - 9.0 * opw * (cs2Q - b.ca2) * b.aH * b.aH / k2);
+        sink(std::integral_constant<int, C_SUP>{}, -opw - 9.0 * opw * (cs2Q - b.ca2) * b.aH * b.aH * ik2);
         sink(std::integral_constant<int, C_UH>{}, -0.5 * opw);
-        T rq = grhom * OmegaDE * b.rhoQ * a2;
-        sink(std::integral_constant<int, C_CH>{}, rq * inv_aH);
+        T rq = grhom * OmegaDE * b.rhoQ * (a * a);
+        sink(std::integral_constant<int, C_CH>{}, rq * iaH);
         sink(std::integral_constant<int, C_C1>{}, -a * (rq + 3.0 * cs2Q * rq));
       }
       break;
@@ -340,9 +369,9 @@ __device__ __forceinline__ void row_visit(uint32_t d, const Bg<T> &b, const Ctx 
         T opw = 1.0 + b.w;
         sink(std::integral_constant<int, C_SUB>{}, cs2Q / opw * k2);
         sink(std::integral_constant<int, C_DIAG>{}, -(1.0 - 3.0 * cs2Q) * b.aH);
-        T rq = grhom * OmegaDE * b.rhoQ * a2;
-        sink(std::integral_constant<int, C_CETA>{}, opw * rq * (0.5 / k2));
-        sink(std::integral_constant<int, C_C1>{}, -a * (3.0 * (cs2Q - b.ca2) * (3.0 * b.aH / k2) * opw * rq));
+        T rq = grhom * OmegaDE * b.rhoQ * (a * a);
+        sink(std::integral_constant<int, C_CETA>{}, opw * rq * hk2);
+        sink(std::integral_constant<int, C_C1>{}, -a * (3.0 * (cs2Q - b.ca2) * (3.0 * b.aH * ik2) * opw * rq));
       }
       break;
     default:
@@ -357,6 +386,9 @@ __device__ __forceinline__ Coef<T> row_coef(uint32_t d, const Bg<T> &b, const Ct
   T v[11];
 #pragma unroll
   for (int i = 0; i < 11; ++i) v[i] = 0.0;
+  // (accumulating rather than assigning keeps the compiler from merging the
+  // cases' stores into one store at a variable index, which would move v to
+  // the stack)
   row_visit<T>(d, b, cx, invtau, [&](auto f, const auto &x) __attribute__((always_inline)) {
     v[decltype(f)::value] = v[decltype(f)::value] + x;
   });

@@ -110,7 +110,7 @@ __device__ __forceinline__ void eval_rhs(Ctx &cx, const uint32_t (&rd)[NS], doub
 // df/dtau: only the hierarchy truncation rows depend on tau explicitly
 template <int NS>
 __device__ __forceinline__ void dfdtau(const uint32_t (&rd)[NS], double tau, const double (&y)[NS], double (&dT)[NS]) {
-  const double it2 = 1.0 / (tau * tau);
+  const double it = 1.0 / tau, it2 = it * it;
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
     uint32_t t = rd_type(rd[s]);
@@ -190,30 +190,34 @@ __device__ __forceinline__ void factor(Mode<NS> &m, Ctx &cx, double tau0, const 
   if (lane == L_TG) Bd[0] -= w_tg_dg * w_dg_tg * m.gh;
   // PCR factorisation; the dark-energy 2x2 block (lanes L_DQ, L_TQ of slot 0)
   // is eliminated at level 0 by cross-multiplication (its theta_Q pivot can
-  // vanish), every other row by the usual division-based reduction
+  // vanish), every other row by the usual reduction, which multiplies by the
+  // neighbours' pivot reciprocals (each lane takes the reciprocal of its own
+  // pivot once per level; rows not coupled to a neighbour never use it)
 #pragma unroll
   for (int L = 0; L < PCR_LEVELS; ++L) {
     if (L >= m.nlev) break;
     const int s = 1 << L;
 #pragma unroll
     for (int j = 0; j < NS; ++j) {
-      double am, bm, cm, ap, bp, cp;
+      const double ib = rcp_nr(Bd[j]);
+      double am, ibm, cm, ap, ibp, cp;
       if (L == 0) {
-        am = from_prev(A[j]); bm = from_prev(Bd[j]); cm = from_prev(Cc[j]);
-        ap = from_next(A[j]); bp = from_next(Bd[j]); cp = from_next(Cc[j]);
+        am = from_prev(A[j]); ibm = from_prev(ib); cm = from_prev(Cc[j]);
+        ap = from_next(A[j]); ibp = from_next(ib); cp = from_next(Cc[j]);
       } else {
         int lm = lane - s < 0 ? 0 : lane - s;
         int lp = lane + s > WAVE - 1 ? WAVE - 1 : lane + s;
-        am = from_lane(A[j], lm); bm = from_lane(Bd[j], lm); cm = from_lane(Cc[j], lm);
-        ap = from_lane(A[j], lp); bp = from_lane(Bd[j], lp); cp = from_lane(Cc[j], lp);
+        am = from_lane(A[j], lm); ibm = from_lane(ib, lm); cm = from_lane(Cc[j], lm);
+        ap = from_lane(A[j], lp); ibp = from_lane(ib, lp); cp = from_lane(Cc[j], lp);
       }
-      double alpha = (A[j] == 0.0) ? 0.0 : -A[j] / bm;
-      double beta = (Cc[j] == 0.0) ? 0.0 : -Cc[j] / bp;
+      double alpha = (A[j] == 0.0) ? 0.0 : -A[j] * ibm;
+      double beta = (Cc[j] == 0.0) ? 0.0 : -Cc[j] * ibp;
       double sg = 1.0;
-      if (L == 0 && j == 0 && (lane == L_DQ || lane == L_TQ)) {
+      if (L == 0 && j == 0) {
         // scale the row by the partner's pivot instead of dividing by it
+        const double bm = from_prev(Bd[0]), bp = from_next(Bd[0]);
         if (lane == L_DQ) { sg = bp; beta = -Cc[j]; alpha = 0.0; }
-        else { sg = bm; alpha = -A[j]; beta = 0.0; }
+        if (lane == L_TQ) { sg = bm; alpha = -A[j]; beta = 0.0; }
       }
       double nb = sg * Bd[j] + alpha * cm + beta * ap;
       double na = alpha * am;
@@ -227,7 +231,7 @@ __device__ __forceinline__ void factor(Mode<NS> &m, Ctx &cx, double tau0, const 
     }
   }
 #pragma unroll
-  for (int j = 0; j < NS; ++j) m.binv[j] = 1.0 / Bd[j];
+  for (int j = 0; j < NS; ++j) m.binv[j] = rcp_nr(Bd[j]);
   // Woodbury: Z = W_B^{-1} U, S = I - C^T Z
 #pragma unroll
   for (int r = 0; r < 3; ++r) {
@@ -340,14 +344,31 @@ template <int NS>
 __device__ __noinline__ void factor_call(Mode<NS> &m, Ctx &cx, double tau0, const double (&y0)[NS], double h) {
   factor<NS>(m, cx, tau0, y0, h);
 }
+// (the calls work on private copies of their array arguments: stores through
+// the references could alias the inputs as far as the compiler knows, which
+// would reload them from the caller's frame after every store)
 template <int NS>
-__device__ __noinline__ void rhs_call(Ctx &cx, const uint32_t (&rd)[NS], double tau, const double (&y)[NS],
-                                      double (&f)[NS]) {
-  eval_rhs<NS, double>(cx, rd, tau, y, f);
+__device__ __noinline__ void rhs_call(Ctx &cx, const uint32_t (&rd)[NS], double tau, const double (&y_in)[NS],
+                                      double (&f_out)[NS]) {
+  double y[NS], f[NS];
+  uint32_t d[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    y[s] = y_in[s];
+    d[s] = rd[s];
+  }
+  eval_rhs<NS, double>(cx, d, tau, y, f);
+#pragma unroll
+  for (int s = 0; s < NS; ++s) f_out[s] = f[s];
 }
 template <int NS>
-__device__ __noinline__ void wsolve_call(const Mode<NS> &m, double (&r)[NS]) {
+__device__ __noinline__ void wsolve_call(const Mode<NS> &m, double (&r_io)[NS]) {
+  double r[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) r[s] = r_io[s];
   wsolve<NS>(m, r);
+#pragma unroll
+  for (int s = 0; s < NS; ++s) r_io[s] = r[s];
 }
 
 // one Rodas5Transformed step (ode_integrators_stiff.py:718-834): y1 and err = k8;

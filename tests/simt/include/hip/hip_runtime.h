@@ -99,6 +99,8 @@ inline uint64_t __ballot(int pred) {
   for (int i = 0; i < 64; ++i) m |= buf[i] << i;
   return m;
 }
+inline double __builtin_amdgcn_rcp(double x) { return 1.0 / x; }
+inline double __builtin_amdgcn_rsq(double x) { return 1.0 / std::sqrt(x); }
 inline void __builtin_amdgcn_s_barrier() { simt::wave->bar.arrive_and_wait(); }
 #define __builtin_amdgcn_fence(order, scope) std::atomic_thread_fence(std::memory_order_seq_cst)
 inline void __threadfence() { std::atomic_thread_fence(std::memory_order_seq_cst); }
@@ -136,6 +138,7 @@ inline T atomicExch(T *p, T v) {
 
 using std::exp;
 using std::fabs;
+using std::fma;
 using std::fmax;
 using std::max;
 using std::min;

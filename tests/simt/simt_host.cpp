@@ -129,10 +129,12 @@ int run(const deb_config_t &cfg, const Layout &L, Reader &in, FILE *out, int mod
   for (int ki = 0; ki < nk; ++ki) {
     std::vector<double> ring(ring_doubles(NS, ckpt), -555.0);
     std::vector<double> KB(7 * NS * WAVE, -666.0), pcr(pcr_lds_doubles<NS>(), -444.0);
+    Ctx cxs;  // the wave's shared context record (every lane stores the same values)
+    std::memset(&cxs, 0xa5, sizeof(cxs));
     // the output conversion's reverse stages its data in the first two blocks
     // of the stage-cotangent buffer, as the kernel does
     run_wave([&](int) {
-      adjoint_mode<NS>(P, A, 0, ki, ring.data(), KB.data(), KB.data(), KB.data() + NS * WAVE, pcr.data());
+      adjoint_mode<NS>(P, A, 0, ki, ring.data(), KB.data(), KB.data(), KB.data() + NS * WAVE, pcr.data(), cxs);
     });
   }
   fwrite(status.data(), 4, nk, out);

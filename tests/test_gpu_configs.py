@@ -94,6 +94,16 @@ def _crosses(theta):
     return (theta[6] + 1.0) * (theta[6] + theta[7] + 1.0) < 0.0
 
 
+def _near_singular(theta):
+    """|1 + w(a)| < 0.02 somewhere in (0, 1] without crossing: the dark-energy
+    equations' c_s^2 / (1 + w) is then large and amplifies roundoff.  Draw 4 of
+    the box (1 + w(1) = -0.009): moving its k by a factor 1 +- 1e-14 moves the
+    oracle's rtol 1e-10 eta' by 1.2e-7 of its largest value (h' 1.5e-8), the
+    other draws' by 1e-12 to 3e-12."""
+    w_end = (theta[6] + 1.0, theta[6] + theta[7] + 1.0)
+    return not _crosses(theta) and min(abs(w_end[0]), abs(w_end[1])) < 0.02
+
+
 MATTER = (4, 6, 8, 10)       # per-mode relative comparisons
 RADIATION = (12, 14)         # decayed by a = 1 at high k: relative to the largest |value| over k
 
@@ -134,9 +144,12 @@ def test_config_matches_oracle_at_rtol_1e5(gpu, name):
             rel = np.abs(y[b, :, -1, f] - ref[:, -1, f]) / (np.abs(ref[:, -1, f]) + 1e-300)
             assert np.all(rel <= fbound), (name, b, f, rel)
             worst_f = max(worst_f, float(rel.max()))
+        # (where w crosses -1 the oracle's own solution is not converged: draw 0
+        # of the box, whose delta_gamma at k = 0.01 differs by 4.0e-4)
+        rbound = np.maximum(fbound, 1e-3) if _crosses(th) else fbound
         for f in RADIATION:
             rel = np.abs(y[b, :, -1, f] - ref[:, -1, f]) / np.abs(ref[:, -1, f]).max()
-            assert np.all(rel <= fbound), (name, b, f, rel)
+            assert np.all(rel <= rbound), (name, b, f, rel)
         if name.startswith("c1"):
             assert np.all(y[b, :, :, 18:20] == 0.0)  # Lambda: delta_Q = theta_Q = 0
     print(f"{name}: rtol 1e-5 GPU vs oracle: P(k) max rel diff {worst_P:.2e} (oracle's own rtol 1e-5 -> 1e-6 "
@@ -170,12 +183,15 @@ def test_config_matches_oracle_at_rtol_1e10(gpu, name):
             rel = np.abs(y[b, :, -1, f] - ref[:, -1, f]) / (np.abs(ref[:, -1, f]) + 1e-300)
             assert rel.max() < 1e-8, (name, b, f, rel.max())
         # eta', h', alpha: on superhorizon scales small residuals of cancelling
-        # sums, so relative to their largest |value| over k; 1e-6 (5.6e-8
-        # measured for draw 4, whose 1 + w(a = 1) = -0.009 makes the dark-energy
-        # rows stiff at late times)
+        # sums, so relative to their largest |value| over k; 1e-6, and 3e-5
+        # where 1 + w comes near 0 (_near_singular: the oracle's roundoff
+        # response there is 1e5 times larger; GPU and oracle, whose roundoff
+        # histories differ at the 1e-12 level, agree to 5.6e-8 .. 1.3e-6 for
+        # draw 4 depending on the build's rounding)
+        mbound = 3e-5 if _near_singular(th) else 1e-6
         for f in (1, 2, 3):
             rel = np.abs(y[b, :, -1, f] - ref[:, -1, f]) / np.abs(ref[:, -1, f]).max()
-            assert rel.max() < 1e-6, (name, b, f, rel.max())
+            assert rel.max() < mbound, (name, b, f, rel.max())
         for f in RADIATION:
             rel = np.abs(y[b, :, -1, f] - ref[:, -1, f]) / np.abs(ref[:, -1, f]).max()
             assert rel.max() < 2e-6, (name, b, f, rel.max())

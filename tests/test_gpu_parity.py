@@ -160,12 +160,16 @@ def test_evolve_parity_tight_tolerance(gpu, test_param):
 
 def _check_final(ref, y, ns, rtol):
     """Final output (a step end point, no interpolation): within 30 rtol.
-    Step counts within 5%: the error estimate k8 is a cancelling sum whose
-    roundoff differs between the structured and the dense solve, which moves
-    step end points by ~1e-8 relative and can flip accept/reject decisions."""
+    Step counts within 12% per mode and 3% in total: the error estimate k8 is a
+    cancelling sum, so roundoff flips accept/reject decisions and the step
+    sequence is chaotic.  The oracle alone shows it: at rtol = atol = 1e-4 on
+    these 24 k, moving every k by a factor 1 +- 1e-15 (or 1e-14, 1e-13) changes
+    single modes' step counts by up to 8.9% (k = 0.3: 372 -> 405) and the total
+    by up to 1.1%."""
     rel = max(_rel(y[:, -1, f], ref["y"][:, -1, f], 0).max() for f in DELTA_FIELDS[:4])
     assert rel < 30 * rtol, rel
-    assert np.all(np.abs(ns - ref["nsteps"]) <= 0.05 * ref["nsteps"] + 2), (ns, ref["nsteps"])
+    assert np.all(np.abs(ns - ref["nsteps"]) <= 0.12 * ref["nsteps"] + 2), (ns, ref["nsteps"])
+    assert abs(int(ns.sum()) - int(ref["nsteps"].sum())) <= 0.03 * ref["nsteps"].sum(), (ns.sum(), ref["nsteps"].sum())
     return rel
 
 
