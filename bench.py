@@ -158,11 +158,38 @@ def copy_calibration():
     print(json.dumps({"copy_bytes_each_way": (1 << 28) * 8}))
 
 
-def _accepted(tape, n_modes):
+def _accepted_per_mode(tape, n_modes):
+    """Accepted steps of every mode from the tape's per-mode state counts."""
     import torch
     off = 256 + 8 * n_modes
     cnt = tape.buf[off:off + 4 * n_modes].view(torch.int32)
-    return int(cnt.sum().item()) - n_modes
+    return cnt.cpu().numpy().astype(np.int64) - 1
+
+
+def _accepted(tape, n_modes):
+    return int(_accepted_per_mode(tape, n_modes).sum())
+
+
+def tail_model(cost, n_cosmo, nk, slots):
+    """The launch's tail: items are dispatched in workgroup order (item i is
+    k index nk-1-i//n_cosmo of cosmology i%n_cosmo: largest k first) to
+    whichever of `slots` resident waves frees up first.  With each item's cost
+    proportional to its step count, the makespan of that list schedule against
+    the ideal (total / slots) gives the fraction of wave-time idle at the end."""
+    import heapq
+    order = [nk - 1 - i // n_cosmo for i in range(n_cosmo * nk)]
+    cosmo = [i % n_cosmo for i in range(n_cosmo * nk)]
+    c = np.asarray(cost, dtype=np.float64).reshape(n_cosmo, nk)
+    heap = [0.0] * min(slots, n_cosmo * nk)
+    heapq.heapify(heap)
+    for ki, ci in zip(order, cosmo):
+        t = heapq.heappop(heap)
+        heapq.heappush(heap, t + c[ci, ki])
+    makespan = max(heap)
+    ideal = c.sum() / slots
+    return {"slots": slots, "makespan_steps": float(makespan), "ideal_steps": float(ideal),
+            "idle_fraction": float(1.0 - ideal / makespan),
+            "model": "greedy dispatch of per-mode step counts in workgroup order, largest k first"}
 
 
 def run_batched(args):
@@ -291,7 +318,11 @@ def run_batched(args):
         "kernel_ms": kern_ms, "modes": int(ns.size), "attempted_steps": attempted, "accepted_steps": accepted,
         "failed_modes": n_fail, "k_modes_per_s": ns.size / (ms_per_step * 1e-3), "setup_s": setup_s,
     }
+    # one wave per SIMD at lmax 32 (512 registers): 4 resident waves per CU
+    slots = 4 * torch.cuda.get_device_properties(dev).multi_processor_count
+    out["tail"] = {"forward": tail_model(ns, nc, nk, slots)}
     if args.arm == "adjoint":
+        out["tail"]["adjoint"] = tail_model(_accepted_per_mode(last["tape"], nc * nk), nc, nk, slots)
         out["tape_bytes"] = last["tape"].nbytes
         if args.plain_sample > 0:
             progress("timing the plain baseline")

@@ -569,63 +569,73 @@ __device__ __forceinline__ void row_feature_adj(uint32_t d, const Ctx &cx, const
 }
 
 // First-order contraction of sum_rows w_row f_row(a, Y) along (a, theta):
-//   row_muls           the multiplier of every coefficient field of every row;
-//   feature_partials_m per lane, the partials of its rows' sum_fields
+//   slot_muls          the multiplier of every coefficient field of a row;
+//   feature_partials   per lane, the partials of its rows' sum_fields
 //                      mul * coef with respect to the features, accumulated
 //                      into fb (plain doubles, no cross-lane sums; linear in
-//                      mul, so multipliers of contractions at the same
-//                      features may be summed first);
+//                      mul, so contractions at the same features may share fb);
 //   lane_dot           after fb is summed over the wave, sum_m dF_m fb_m for
 //                      the lane's own tangent direction (lane_ctx).
 // The functionals (h', eta', Pi, f_1) enter through their double values in rs,
 // which is exact at first order: d(u H) = du H + u dH with dH = sum_rows dc_h Y,
 // so sum_rows w u dH = s_h sum_rows dc_h Y (multiplier Y s_h of the c_h field).
 
-// multipliers of the coefficient fields of every row in sum_rows w f_row(Y)
-// (the contraction's per-row weights; rs: the functionals' values)
-template <int NS, class T = double>
-__device__ __forceinline__ void row_muls(const double (&Y)[NS], const double (&w)[NS], const RowSumsT<T> &rs,
-                                         T (&mul)[NS][11]) {
+// multipliers of the coefficient fields of slot s's rows in
+// sum_rows w f_row(Y) (the contraction's per-row weights): ys, ws the slot's
+// values, ym, yp its neighbours' (lanes i-1, i+1), ydg, ytg, wahp the
+// wave-uniform Y[L_DG], Y[L_TG], w[L_AHP] of slot 0; rs the functionals' values
+template <class T>
+__device__ __forceinline__ void slot_muls(int s, double ys, double ws, double ym, double yp, double ydg, double ytg,
+                                          double wahp, const RowSumsT<T> &rs, T (&ms)[11]) {
   const int lane = lane_id();
-  const double ydg = readlane(Y[0], L_DG), ytg = readlane(Y[0], L_TG);
-  const double wahp = readlane(w[0], L_AHP);
-#pragma unroll
-  for (int s = 0; s < NS; ++s) {
-    const double ys = Y[s], ws = w[s];
-    const double ym = from_prev(ys), yp = from_next(ys);
-    const double lp = (s == 0) ? (lane == L_DG ? ytg : (lane == L_TG ? ydg : 0.0)) : 0.0;
-    mul[s][C_SUB] = tconst<T>(ws * ym);
-    mul[s][C_DIAG] = tconst<T>(ws * ys);
-    mul[s][C_SUP] = tconst<T>(ws * yp);
-    mul[s][C_UH] = ws * rs.H;
-    mul[s][C_UETA] = ws * rs.E;
-    mul[s][C_UPI] = ws * rs.Pi;
-    mul[s][C_CH] = ys * rs.sh;
-    mul[s][C_CETA] = ys * rs.se;
-    mul[s][C_CPI] = ys * rs.sp;
-    mul[s][C_C1] = tconst<T>(ys * wahp);
-    // the leaf field is unused by the a-row: there it carries the row weight w_a
-    mul[s][C_LEAF] = tconst<T>((s == 0 && lane == L_A) ? ws : ws * lp);
-  }
+  const double lp = (s == 0) ? (lane == L_DG ? ytg : (lane == L_TG ? ydg : 0.0)) : 0.0;
+  ms[C_SUB] = tconst<T>(ws * ym);
+  ms[C_DIAG] = tconst<T>(ws * ys);
+  ms[C_SUP] = tconst<T>(ws * yp);
+  ms[C_UH] = ws * rs.H;
+  ms[C_UETA] = ws * rs.E;
+  ms[C_UPI] = ws * rs.Pi;
+  ms[C_CH] = ys * rs.sh;
+  ms[C_CETA] = ys * rs.se;
+  ms[C_CPI] = ys * rs.sp;
+  ms[C_C1] = tconst<T>(ys * wahp);
+  // the leaf field is unused by the a-row: there it carries the row weight w_a
+  ms[C_LEAF] = tconst<T>((s == 0 && lane == L_A) ? ws : ws * lp);
 }
 
-// fb[q] += partials of sum_rows sum_fields mul[q][row][field] coef_field with
-// respect to the features fv[q] (K contractions in one pass over the rows)
+// fb[q] += partials, with respect to the features fv[q], of
+// sum_rows sum_fields mul_q coef_field, mul_q the row multipliers of
+// sum_rows w[q] f_row(Y[q]) (slot_muls, formed slot by slot here; rs[q] the
+// functionals' values): K contractions in one pass of the divergent row switch
 template <int NS, int K, class T = double>
-DEB_OUTLINE void feature_partials_m(const Ctx &cx, const uint32_t (&rd)[NS], const T (&fv_in)[K][FT_N],
-                                    const T (&mul)[K][NS][11], T (&fb)[K][FT_N]) {
+DEB_OUTLINE void feature_partials(const Ctx &cx, const uint32_t (&rd)[NS], const T (&fv_in)[K][FT_N],
+                                  const double (&Y_in)[K][NS], const double (&w_in)[K][NS],
+                                  const RowSumsT<T> (&rs_in)[K], T (&fb)[K][FT_N]) {
   const int lane = lane_id();
-  // Private copies: fb may alias fv and mul as far as the compiler knows, so
+  // Private copies: fb may alias the inputs as far as the compiler knows, so
   // accumulating through the references would reload every operand after each
   // store (memory traffic in the caller's frame); locals stay in registers.
   T fv[K][FT_N], acc[K][FT_N];
+  double Y[K][NS], w[K][NS];
+  RowSumsT<T> rs[K];
+  double ydg[K], ytg[K], wahp[K];
 #pragma unroll
-  for (int q = 0; q < K; ++q)
+  for (int q = 0; q < K; ++q) {
 #pragma unroll
     for (int m = 0; m < FT_N; ++m) {
       fv[q][m] = fv_in[q][m];
       acc[q][m] = tconst<T>(0.0);
     }
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      Y[q][s] = Y_in[q][s];
+      w[q][s] = w_in[q][s];
+    }
+    rs[q] = rs_in[q];
+    ydg[q] = readlane(Y[q][0], L_DG);
+    ytg[q] = readlane(Y[q][0], L_TG);
+    wahp[q] = readlane(w[q][0], L_AHP);
+  }
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
     const uint32_t ty = rd_type(rd[s]);
@@ -634,8 +644,8 @@ DEB_OUTLINE void feature_partials_m(const Ctx &cx, const uint32_t (&rd)[NS], con
     T ms[K][11], vq[K], xq[K], vb[K];
 #pragma unroll
     for (int q = 0; q < K; ++q) {
-#pragma unroll
-      for (int f = 0; f < 11; ++f) ms[q][f] = mul[q][s][f];
+      slot_muls<T>(s, Y[q][s], w[q][s], from_prev(Y[q][s]), from_next(Y[q][s]), ydg[q], ytg[q], wahp[q], rs[q],
+                   ms[q]);
       // v_q(x), x = a amnu / q
       xq[q] = fv[q][FT_A] * fv[q][FT_AMNU] * iq;
       vq[q] = rsqrt_t<T>(1.0 + xq[q] * xq[q]);
@@ -651,11 +661,11 @@ DEB_OUTLINE void feature_partials_m(const Ctx &cx, const uint32_t (&rd)[NS], con
       }
     }
   }
-  if (lane == L_A) {  // f_a = aH a, weight w_a (row_muls)
+  if (lane == L_A) {  // f_a = aH a, weight w_a (the leaf multiplier of the a-row)
 #pragma unroll
     for (int q = 0; q < K; ++q) {
-      acc[q][FT_AH] += mul[q][0][C_LEAF] * fv[q][FT_A];
-      acc[q][FT_A] += mul[q][0][C_LEAF] * fv[q][FT_AH];
+      acc[q][FT_AH] += w[q][0] * fv[q][FT_A];
+      acc[q][FT_A] += w[q][0] * fv[q][FT_AH];
     }
   }
 #pragma unroll
@@ -870,15 +880,14 @@ __device__ __forceinline__ void rodas5_step_vjp(Mode<NS> &m, Ctx &cx, double t0,
 #pragma unroll
   for (int j = 0; j < MAXDIR; ++j) th_acc[j] = 0.0;
   double a0b = 0.0, t0b = 0.0, hb = 0.0;
-  // Feature partials: per stage for f(t_i, u_i) (features at a_i); the
-  // J(t0, y0) k_i terms share the features at a0 and are linear in their row
-  // multipliers, which are summed over the stages and passed over once.
-  double fvJ[1][FT_N], mulJ[1][NS][11];
-  feature_values(cx, a0, fvJ[0]);
+  // Feature partials, one row pass per stage with two contractions: f(t_i, u_i)
+  // at the stage's features (fvs[0], fbs[0], contracted every stage) and the
+  // J(t0, y0) k_i term at the features of a0 (fvs[1]; its partials fbs[1]
+  // accumulate over the stages and are contracted once).
+  double fvs[2][FT_N], fbs[2][FT_N];
+  feature_values(cx, a0, fvs[1]);
 #pragma unroll
-  for (int q = 0; q < NS; ++q)
-#pragma unroll
-    for (int f = 0; f < 11; ++f) mulJ[0][q][f] = 0.0;
+  for (int q = 0; q < FT_N; ++q) fbs[1][q] = 0.0;
 #pragma nounroll
   for (int i = 7; i >= 0; --i) {
     double rb[NS], ki[NS];
@@ -928,24 +937,25 @@ __device__ __forceinline__ void rodas5_step_vjp(Mode<NS> &m, Ctx &cx, double t0,
     double ub[NS];
     {
       const double ai = readlane(u[0], L_A);
-      double fv[1][FT_N], fb[1][FT_N], mul[1][NS][11];
-      const RowSums rs = coef_pass<NS, true>(cx, m.rd, ai, iti, rb, u, ub, fv[0]);
-      row_muls<NS>(u, rb, rs, mul[0]);
+      double Yq[2][NS], wq[2][NS];
+      RowSums rsq[2];
+      rsq[0] = coef_pass<NS, true>(cx, m.rd, ai, iti, rb, u, ub, fvs[0]);
 #pragma unroll
-      for (int q = 0; q < FT_N; ++q) fb[0][q] = 0.0;
-      feature_partials_m<NS, 1>(cx, m.rd, fv, mul, fb);
-      {
-        double wr[NS], mr[NS][11];
-#pragma unroll
-        for (int s = 0; s < NS; ++s) wr[s] = (s == 0 && lane == L_A) ? 0.0 : rb[s];
-        row_muls<NS>(ki, wr, jac_sums<NS>(m, ki, wr), mr);
-#pragma unroll
-        for (int s = 0; s < NS; ++s)
-#pragma unroll
-          for (int f = 0; f < 11; ++f) mulJ[0][s][f] += mr[s][f];
+      for (int s = 0; s < NS; ++s) {
+        Yq[0][s] = u[s];
+        wq[0][s] = rb[s];
+        Yq[1][s] = ki[s];
+        wq[1][s] = (s == 0 && lane == L_A) ? 0.0 : rb[s];
       }
-      wave_sum<FT_N>(fb[0]);
-      const double g = lane_contract(cx, ai, fb[0]);
+      rsq[1] = jac_sums<NS>(m, ki, wq[1]);
+#pragma unroll
+      for (int q = 0; q < FT_N; ++q) fbs[0][q] = 0.0;
+      feature_partials<NS, 2>(cx, m.rd, fvs, Yq, wq, rsq, fbs);
+      double fb[FT_N];
+#pragma unroll
+      for (int q = 0; q < FT_N; ++q) fb[q] = fbs[0][q];
+      wave_sum<FT_N>(fb);
+      const double g = lane_contract(cx, ai, fb);
       const double ga = readlane(g, 0);
       if (lane == L_A) ub[0] += ga;
       double tl[MAXDIR];
@@ -966,12 +976,11 @@ __device__ __forceinline__ void rodas5_step_vjp(Mode<NS> &m, Ctx &cx, double t0,
   }
   // the J(t0, y0) k_i terms, contracted with the feature tangents at a0
   {
-    double fbJ[1][FT_N];
+    double fbJ[FT_N];
 #pragma unroll
-    for (int q = 0; q < FT_N; ++q) fbJ[0][q] = 0.0;
-    feature_partials_m<NS, 1>(cx, m.rd, fvJ, mulJ, fbJ);
-    wave_sum<FT_N>(fbJ[0]);
-    const double g = lane_contract(cx, a0, fbJ[0]);
+    for (int q = 0; q < FT_N; ++q) fbJ[q] = fbs[1][q];
+    wave_sum<FT_N>(fbJ);
+    const double g = lane_contract(cx, a0, fbJ);
     a0b += readlane(g, 0);
     double tl[MAXDIR];
     lane_gather(g, cx.ndir, tl);
@@ -989,14 +998,19 @@ __device__ __forceinline__ void rodas5_step_vjp(Mode<NS> &m, Ctx &cx, double t0,
     for (int s = 0; s < NS; ++s) y0b[s] += gy[s].d;
     // S = rho^T f(a, y0): feature partials P and their a-derivative Q in one
     // pass in dual arithmetic along a, then the features' second derivatives
-    dual fvd[1][FT_N], fbd[1][FT_N], muld[1][NS][11];
+    dual fvd[1][FT_N], fbd[1][FT_N];
     features_call<dual>(cx, ad, fvd[0]);
-    RowSumsT<dual> rsd;
-    row_sums_t<NS, dual>(cx, m.rd, ad, it0, rho, y0, rsd);
-    row_muls<NS, dual>(y0, rho, rsd, muld[0]);
+    RowSumsT<dual> rsd[1];
+    row_sums_t<NS, dual>(cx, m.rd, ad, it0, rho, y0, rsd[0]);
+    double Yd[1][NS], wd[1][NS];
 #pragma unroll
-    for (int q = 0; q < FT_N; ++q) fbd[0][q] = 0.0;
-    feature_partials_m<NS, 1, dual>(cx, m.rd, fvd, muld, fbd);
+    for (int s = 0; s < NS; ++s) {
+      Yd[0][s] = y0[s];
+      wd[0][s] = rho[s];
+    }
+#pragma unroll
+    for (int q = 0; q < FT_N; ++q) fbd[0][q] = tconst<dual>(0.0);
+    feature_partials<NS, 1, dual>(cx, m.rd, fvd, Yd, wd, rsd, fbd);
     double PQ[2 * FT_N];
 #pragma unroll
     for (int q = 0; q < FT_N; ++q) {

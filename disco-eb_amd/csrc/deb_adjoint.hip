@@ -102,12 +102,17 @@ __global__ void __launch_bounds__(WAVE) deb_rhs_vjp_kernel(const Params<NS> P, c
   }
   const double tau = H.t0[item], it = 1.0 / tau;
   const double a = readlane(y[0], L_A);
-  double fv[1][FT_N], fb[1][FT_N], mul[1][NS][11];
-  const RowSums rs = coef_pass<NS, true>(cx, rd, a, it, w, y, yb, fv[0]);
-  row_muls<NS>(y, w, rs, mul[0]);
+  double fv[1][FT_N], fb[1][FT_N], Yq[1][NS], wq[1][NS];
+  RowSums rs[1];
+  rs[0] = coef_pass<NS, true>(cx, rd, a, it, w, y, yb, fv[0]);
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    Yq[0][s] = y[s];
+    wq[0][s] = w[s];
+  }
 #pragma unroll
   for (int q = 0; q < FT_N; ++q) fb[0][q] = 0.0;
-  feature_partials_m<NS, 1>(cx, rd, fv, mul, fb);
+  feature_partials<NS, 1>(cx, rd, fv, Yq, wq, rs, fb);
   wave_sum<FT_N>(fb[0]);
   const double gl = lane_contract(cx, a, fb[0]);
   double gth[MAXDIR];
