@@ -187,8 +187,14 @@ def tail_model(cost, n_cosmo, nk, slots):
         heapq.heappush(heap, t + c[ci, ki])
     makespan = max(heap)
     ideal = c.sum() / slots
+    # the same dispatch with the items sorted by their step counts (longest first)
+    heap = [0.0] * min(slots, n_cosmo * nk)
+    for v in np.sort(c.ravel())[::-1]:
+        t = heapq.heappop(heap)
+        heapq.heappush(heap, t + v)
     return {"slots": slots, "makespan_steps": float(makespan), "ideal_steps": float(ideal),
-            "idle_fraction": float(1.0 - ideal / makespan),
+            "idle_fraction": float(1.0 - ideal / makespan), "longest_item_steps": float(c.max()),
+            "sorted_makespan_steps": float(max(heap)),
             "model": "greedy dispatch of per-mode step counts in workgroup order, largest k first"}
 
 
@@ -429,7 +435,7 @@ def run_loss_adjoint(args):
                              "accepted steps (each recomputed from its checkpoint and reversed)"),
         "kernel_ms": kern_ms, "modes": int(st.size), "attempted_steps": float(last["ns"].sum().item()),
         "accepted_steps": accepted, "tape_bytes": last["tape"].nbytes,
-        "adjoint_workspace_bytes": gradient.adjoint_workspace_bytes(cfg, args.ckpt_every), "setup_s": setup_s,
+        "adjoint_workspace_bytes": gradient.adjoint_workspace_bytes(cfg, nc, nk, args.ckpt_every), "setup_s": setup_s,
         "loss_mean": float(last["loss"].mean().item()),
     }
     print(json.dumps(out))

@@ -1170,6 +1170,7 @@ struct AdjArgs {
   TapeView tape;
   double *rings;          // [n_rings][ring_doubles(NS, C)] reverse-launch rings
   int32_t *ring_busy;     // [n_rings] 1 while a wave owns the ring
+  const int32_t *order;   // [n_modes] launch order (mode indices, longest first), or null
   int n_rings;
   int n_in;               // state entries read by the output conversion
   int32_t in_idx[WAVE];   // their reference indices, one per lane

@@ -35,8 +35,15 @@ __global__ void __launch_bounds__(WAVE) deb_adjoint_kernel(const Params<NS> P, c
   __shared__ Ctx cxs;
   const int item = blockIdx.x;
   if (item >= P.n_cosmo * P.nk) return;
-  const int ki = P.nk - 1 - item / P.n_cosmo;
-  const int ci = item % P.n_cosmo;
+  int ki, ci;
+  if (A.order) {  // modes by accepted steps, longest first
+    const int mode = A.order[item];
+    ci = mode / P.nk;
+    ki = mode % P.nk;
+  } else {
+    ki = P.nk - 1 - item / P.n_cosmo;
+    ci = item % P.n_cosmo;
+  }
   const int lane = lane_id();
   int ring = -1;
   if (lane == 0) {
@@ -178,6 +185,60 @@ __global__ void __launch_bounds__(WAVE) deb_step_vjp_kernel(const Params<NS> P, 
 }
 
 // ---------------------------------------------------------------------------
+// launch order of the reverse launch.  Each mode's cost is proportional to its
+// accepted steps, which the tape records; handing the longest modes out first
+// (longest-processing-time order) shortens the launch's tail, where the last
+// long modes run on a few SIMDs while the rest of the GPU idles.  A bucket
+// sort over 8-step buckets (the last one open-ended): histogram, exclusive
+// scan from the longest bucket down, scatter.  Ties within a bucket take any
+// order; each mode's result does not depend on it.
+constexpr int ORDER_BUCKETS = 8192;
+
+__device__ __forceinline__ int order_bucket(int32_t count) {
+  int b = count < 0 ? 0 : (count >> 3);
+  b = b > ORDER_BUCKETS - 1 ? ORDER_BUCKETS - 1 : b;
+  return ORDER_BUCKETS - 1 - b;  // bucket 0 holds the longest modes
+}
+
+__global__ void __launch_bounds__(256) deb_order_hist_kernel(const int32_t *count, long long n, int32_t *hist) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) atomicAdd(&hist[order_bucket(count[i])], 1);
+}
+
+// exclusive prefix sums of the histogram, in place (one block of 1024 threads)
+__global__ void __launch_bounds__(1024) deb_order_scan_kernel(int32_t *hist) {
+  constexpr int PER = ORDER_BUCKETS / 1024;
+  __shared__ int32_t part[1024];
+  const int t = threadIdx.x;
+  int32_t loc[PER], sum = 0;
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    loc[j] = hist[t * PER + j];
+    sum += loc[j];
+  }
+  part[t] = sum;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const int32_t v = t >= off ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int32_t run = part[t] - sum;
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    hist[t * PER + j] = run;
+    run += loc[j];
+  }
+}
+
+__global__ void __launch_bounds__(256) deb_order_scatter_kernel(const int32_t *count, long long n, int32_t *offs,
+                                                                 int32_t *order) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) order[atomicAdd(&offs[order_bucket(count[i])], 1)] = (int32_t)i;
+}
+
+// ---------------------------------------------------------------------------
 // host side
 
 // rings of the reverse launch: twice the waves the device can hold at the
@@ -196,16 +257,23 @@ int adjoint_rings(int &n_rings) {
   return 0;
 }
 
-inline long long adjoint_ws_bytes(int ns, int ckpt, int n_rings) {
-  const long long flags = ((long long)n_rings * 4 + 255) / 256 * 256;
-  return flags + (long long)n_rings * ring_doubles(ns, ckpt) * 8;
+// workspace: [ring busy flags][order histogram][launch order][rings], each
+// part 256-byte aligned
+inline long long round256(long long b) { return (b + 255) / 256 * 256; }
+inline long long adjoint_hist_off(int n_rings) { return round256((long long)n_rings * 4); }
+inline long long adjoint_order_off(int n_rings) { return adjoint_hist_off(n_rings) + round256(4LL * ORDER_BUCKETS); }
+inline long long adjoint_rings_off(int n_rings, long long n_modes) {
+  return adjoint_order_off(n_rings) + round256(4 * n_modes);
+}
+inline long long adjoint_ws_bytes(int ns, int ckpt, int n_rings, long long n_modes) {
+  return adjoint_rings_off(n_rings, n_modes) + (long long)n_rings * ring_doubles(ns, ckpt) * 8;
 }
 
 template <int NS>
-int adjoint_workspace(int ckpt, size_t *bytes) {
+int adjoint_workspace(int ckpt, long long n_modes, size_t *bytes) {
   int n_rings = 0;
   if (int r = adjoint_rings<NS>(n_rings)) return r;
-  *bytes = (size_t)adjoint_ws_bytes(NS, ckpt, n_rings);
+  *bytes = (size_t)adjoint_ws_bytes(NS, ckpt, n_rings, n_modes);
   return 0;
 }
 
@@ -230,15 +298,31 @@ int run_adjoint(const deb_config_t *cfg, const Layout &L, int n_cosmo, const deb
   if (int r = tape_view(A.tape, const_cast<void *>(tape), tape_bytes, (long long)n_cosmo * nk, NS, ckpt)) return r;
   int n_rings = 0;
   if (int r = adjoint_rings<NS>(n_rings)) return r;
-  const long long need = adjoint_ws_bytes(NS, ckpt, n_rings);
+  const long long n_modes = (long long)n_cosmo * nk;
+  const long long need = adjoint_ws_bytes(NS, ckpt, n_rings, n_modes);
   if (!ws || (long long)ws_bytes < need) return fail(DEB_E_ARG, "workspace smaller than deb_adjoint_workspace_bytes");
-  A.ring_busy = static_cast<int32_t *>(ws);
-  A.rings = reinterpret_cast<double *>(static_cast<char *>(ws) + (need - (long long)n_rings * ring_doubles(NS, ckpt) * 8));
+  char *wb = static_cast<char *>(ws);
+  A.ring_busy = reinterpret_cast<int32_t *>(wb);
+  int32_t *hist = reinterpret_cast<int32_t *>(wb + adjoint_hist_off(n_rings));
+  int32_t *order = reinterpret_cast<int32_t *>(wb + adjoint_order_off(n_rings));
+  A.order = order;
+  A.rings = reinterpret_cast<double *>(wb + adjoint_rings_off(n_rings, n_modes));
   A.n_rings = n_rings;
   A.n_in = output_inputs(*cfg, A.in_idx);
+  // (flags and histogram cleared by kernels: no memset nodes if the caller
+  // captures the launches in a graph)
   hipLaunchKernelGGL(deb_ring_reset_kernel, dim3((n_rings + 255) / 256), dim3(256), 0, st, A.ring_busy, n_rings);
   if (int r = hip_check(hipGetLastError(), "ring reset launch")) return r;
-  hipLaunchKernelGGL(deb_adjoint_kernel<NS>, dim3((unsigned)((int64_t)n_cosmo * nk)), dim3(WAVE), 0, st, P, A);
+  hipLaunchKernelGGL(deb_ring_reset_kernel, dim3(ORDER_BUCKETS / 256), dim3(256), 0, st, hist, ORDER_BUCKETS);
+  if (int r = hip_check(hipGetLastError(), "histogram reset launch")) return r;
+  const unsigned nb = (unsigned)((n_modes + 255) / 256);
+  hipLaunchKernelGGL(deb_order_hist_kernel, dim3(nb), dim3(256), 0, st, A.tape.count, n_modes, hist);
+  if (int r = hip_check(hipGetLastError(), "order histogram launch")) return r;
+  hipLaunchKernelGGL(deb_order_scan_kernel, dim3(1), dim3(1024), 0, st, hist);
+  if (int r = hip_check(hipGetLastError(), "order scan launch")) return r;
+  hipLaunchKernelGGL(deb_order_scatter_kernel, dim3(nb), dim3(256), 0, st, A.tape.count, n_modes, hist, order);
+  if (int r = hip_check(hipGetLastError(), "order scatter launch")) return r;
+  hipLaunchKernelGGL(deb_adjoint_kernel<NS>, dim3((unsigned)n_modes), dim3(WAVE), 0, st, P, A);
   return hip_check(hipGetLastError(), "adjoint kernel launch");
 }
 
@@ -277,12 +361,13 @@ int deb_tape_bytes(const deb_config_t *cfg, int n_cosmo, int nk, int64_t n_state
   return 0;
 }
 
-int deb_adjoint_workspace_bytes(const deb_config_t *cfg, int ckpt_every, size_t *bytes) {
-  if (!cfg || !bytes) return fail(DEB_E_ARG, "invalid argument");
+int deb_adjoint_workspace_bytes(const deb_config_t *cfg, int n_cosmo, int nk, int ckpt_every, size_t *bytes) {
+  if (!cfg || !bytes || n_cosmo < 1 || nk < 1) return fail(DEB_E_ARG, "invalid argument");
+  if ((long long)n_cosmo * nk > 0x7fffffffLL) return fail(DEB_E_ARG, "more than 2^31 - 1 modes");
   if (ckpt_every < 1 || ckpt_every > TAPE_MAX_CKPT) return fail(DEB_E_ARG, "checkpoint interval outside [1, 4096]");
   Layout L;
   if (int r = build_layout(*cfg, L)) return r;
-  DEB_DISPATCH(L.ns, (adjoint_workspace<NS>(ckpt_every, bytes)))
+  DEB_DISPATCH(L.ns, (adjoint_workspace<NS>(ckpt_every, (long long)n_cosmo * nk, bytes)))
 }
 
 int deb_solve_adjoint(const deb_config_t *cfg, int n_cosmo, const deb_cosmo_t *cosmo, const double *tables,

@@ -39,6 +39,19 @@ constexpr int pcr_lds_doubles() {
   return PCR_LEVELS * 2 * NS * WAVE;
 }
 
+// the forward kernels keep the step's 7 stage vectors in LDS where the wave's
+// share of it allows (four waves per CU: 40 KB each, with the PCR
+// coefficients), else in registers; the same LDS stages the output fields
+// between steps
+template <int NS>
+constexpr bool stage_in_lds() {
+  return (7 * NS * WAVE + pcr_lds_doubles<NS>()) * 8 <= 40960;
+}
+template <int NS>
+constexpr int fwd_lds_doubles() {
+  return stage_in_lds<NS>() ? 7 * NS * WAVE : NS * WAVE;
+}
+
 // row types (lane roles)
 enum : uint32_t {
   R_NONE = 0,
@@ -501,28 +514,62 @@ __device__ __forceinline__ T spl_piece(const Spl &s, int i, T xv) {
   return A * y0 + B * y1 + ((A * A * A - A) * s0 + (B * B * B - B) * s1) * (h * h / 6.0);
 }
 
-// three splines at one abscissa (the same intervals as spl_interval): the
-// hinted intervals' end knots of all three are loaded before any is tested,
-// so their latencies overlap
+// a wave-uniform pointer moved to scalar registers, so that loads through it
+// with a wave-uniform index can go through the scalar cache
+__device__ __forceinline__ const double *uniform_ptr(const double *p) {
+  const uint64_t v = (uint64_t)(uintptr_t)p;
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+  return (const double *)(uintptr_t)(((uint64_t)hi << 32) | lo);
+}
+
+// cubic spline piece from preloaded knots (x0, x1, y0, y1, S0, S1)
+template <typename T>
+__device__ __forceinline__ T spl_piece_k(const double (&kn)[6], T xv) {
+  const double h = kn[1] - kn[0];
+  T t = (xv - kn[0]) / h;
+  T A = 1.0 - t;
+  T B = t;
+  return A * kn[2] + B * kn[3] + ((A * A * A - A) * kn[4] + (B * B * B - B) * kn[5]) * (h * h / 6.0);
+}
+
+// three splines at one wave-uniform abscissa (the same intervals as
+// spl_interval): all six knots of the three hinted intervals are loaded
+// before any is tested (one round trip when the hints hold, which is the rule
+// between the stages of a step); a missed hint searches and reloads
 template <typename T>
 __device__ __forceinline__ void spl_eval3(const Spl (&sp)[3], T xv, int *const (&hint)[3], T (&r)[3]) {
   const double v = vval(xv);
   int i[3];
-  double lo[3], hi[3];
+  double kn[3][6];
 #pragma unroll
   for (int j = 0; j < 3; ++j) {
     i[j] = *hint[j];
-    const int c = i[j] < 0 ? 0 : (i[j] > sp[j].n - 2 ? sp[j].n - 2 : i[j]);
-    lo[j] = sp[j].x[c];
-    hi[j] = sp[j].x[c + 1];
+    int c = i[j] < 0 ? 0 : (i[j] > sp[j].n - 2 ? sp[j].n - 2 : i[j]);
+    c = __builtin_amdgcn_readfirstlane(c);
+    const double *x = uniform_ptr(sp[j].x), *y = uniform_ptr(sp[j].y), *S = uniform_ptr(sp[j].S);
+    kn[j][0] = x[c];
+    kn[j][1] = x[c + 1];
+    kn[j][2] = y[c];
+    kn[j][3] = y[c + 1];
+    kn[j][4] = S[c];
+    kn[j][5] = S[c + 1];
   }
 #pragma unroll
   for (int j = 0; j < 3; ++j) {
-    const bool ok = i[j] >= 0 && i[j] <= sp[j].n - 2 && lo[j] < v && v <= hi[j];
-    if (!ok) i[j] = spl_interval(sp[j], v, *hint[j]);
+    const bool ok = i[j] >= 0 && i[j] <= sp[j].n - 2 && kn[j][0] < v && v <= kn[j][1];
+    if (!ok) {
+      const int c = spl_interval(sp[j], v, *hint[j]);
+      kn[j][0] = sp[j].x[c];
+      kn[j][1] = sp[j].x[c + 1];
+      kn[j][2] = sp[j].y[c];
+      kn[j][3] = sp[j].y[c + 1];
+      kn[j][4] = sp[j].S[c];
+      kn[j][5] = sp[j].S[c + 1];
+    }
   }
 #pragma unroll
-  for (int j = 0; j < 3; ++j) r[j] = spl_piece(sp[j], i[j], xv);
+  for (int j = 0; j < 3; ++j) r[j] = spl_piece_k(kn[j], xv);
 }
 
 template <typename T>

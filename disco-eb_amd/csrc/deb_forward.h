@@ -300,6 +300,8 @@ __device__ __forceinline__ void integrate_mode(const Params<NS> &P, int ci, int 
 #pragma unroll
   for (int s = 0; s < NS; ++s) m.rd[s] = P.rowdesc[s][lane];
   m.nlev = mode_levels(P.cfg, P.nq);
+  // stage vectors in the kernel's LDS (lds holds fwd_lds_doubles<NS>())
+  lds_double *const kl = stage_in_lds<NS>() ? (lds_double *)lds : nullptr;
   const double *tout = P.tau_out + (int64_t)ci * P.nout;
   double tmin = tout[0], tmax = tout[0];
   for (int i = 1; i < P.nout; ++i) {
@@ -342,7 +344,7 @@ __device__ __forceinline__ void integrate_mode(const Params<NS> &P, int ci, int 
       break;
     }
     const double dt = tnext - tprev;
-    rodas5_step<NS>(m, cx, tprev, dt, y, y1, err);
+    rodas5_step<NS>(m, cx, tprev, dt, y, y1, err, kl);
     // filtered, weighted RMS of the scaled error: lanes outside the filter
     // never enter it, so a non-finite unfiltered entry cannot poison it
     const double sc = err[0] / (cf.atol + fmax(fabs(y[0]), fabs(y1[0])) * cf.rtol);
@@ -414,6 +416,8 @@ __device__ __forceinline__ void integrate_batch(const Params<NS> &P, int ci, int
 #pragma unroll
   for (int s = 0; s < NS; ++s) m.rd[s] = P.rowdesc[s][lane];
   m.nlev = mode_levels(P.cfg, P.nq);
+  // stage vectors in the kernel's LDS (lds holds fwd_lds_doubles<NS>())
+  lds_double *const kl = stage_in_lds<NS>() ? (lds_double *)lds : nullptr;
   auto state = [&](int b, int which) -> double * {
     return ws + ((mode0 + b) * 2 + which) * (int64_t)(NS * WAVE) + lane;
   };
@@ -462,7 +466,7 @@ __device__ __forceinline__ void integrate_batch(const Params<NS> &P, int ci, int
       const double *src = state(b, 0);
 #pragma unroll
       for (int s = 0; s < NS; ++s) y[s] = src[s * WAVE];
-      rodas5_step<NS>(m, cx, tprev, dt, y, y1, err);
+      rodas5_step<NS>(m, cx, tprev, dt, y, y1, err, kl);
       // error-norm weights: indices (a, eta, delta_c, delta_b, theta_b, delta_gamma) of slot 0
       double ew = 0.0;
       if (lane == L_A || lane == L_DC || lane == L_DB || lane == L_DG) ew = 1.0;
@@ -539,7 +543,7 @@ __device__ __forceinline__ void integrate_batch(const Params<NS> &P, int ci, int
 // are numbered so that the largest k (most steps) start first.
 template <int NS>
 __global__ void __launch_bounds__(WAVE) deb_forward_kernel(const Params<NS> P) {
-  __shared__ double lds[WAVE * NS];
+  __shared__ double lds[fwd_lds_doubles<NS>()];
   __shared__ double pcr[pcr_lds_doubles<NS>()];
   const int item = blockIdx.x;
   if (item >= P.n_cosmo * P.nk) return;

@@ -65,7 +65,7 @@ static void *scratch_ptr() {
 // consecutive k), groups of the largest k first.
 template <int NS>
 __global__ void __launch_bounds__(WAVE) deb_forward_batched_kernel(const Params<NS> P, const int B, double *ws) {
-  __shared__ double lds[WAVE * NS];
+  __shared__ double lds[fwd_lds_doubles<NS>()];
   __shared__ double pcr[pcr_lds_doubles<NS>()];
   const int ng = P.nk / B;
   const int item = blockIdx.x;
@@ -79,7 +79,7 @@ __global__ void __launch_bounds__(WAVE) deb_forward_batched_kernel(const Params<
 // tape of the gradient path (deb_solve_forward_tape).
 template <int NS>
 __global__ void __launch_bounds__(WAVE) deb_forward_tape_kernel(const Params<NS> P, const TapeView T) {
-  __shared__ double lds[WAVE * NS];
+  __shared__ double lds[fwd_lds_doubles<NS>()];
   __shared__ double pcr[pcr_lds_doubles<NS>()];
   const int item = blockIdx.x;
   if (item >= P.n_cosmo * P.nk) return;

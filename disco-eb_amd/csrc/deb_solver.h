@@ -373,11 +373,16 @@ __device__ __noinline__ void wsolve_call(const Mode<NS> &m, double (&r_io)[NS]) 
 
 // one Rodas5Transformed step (ode_integrators_stiff.py:718-834): y1 and err = k8;
 // st(i, k) sees every stage vector k_{i+1} once it is solved.  CALLS selects
-// the out-of-line factorisation, right-hand side and solve.
+// the out-of-line factorisation, right-hand side and solve.  With kl (the
+// wave's LDS, 7 x NS x 64 doubles) the stage vectors k1..k7 are kept there
+// instead of in registers, which relieves the register file of the kernels
+// that run the step fully inlined.
 template <int NS, class Store, bool CALLS = false>
 __device__ __forceinline__ void rodas5_step_s(Mode<NS> &m, Ctx &cx, double t0, double dt, const double (&y0)[NS],
-                                              double (&y1)[NS], double (&err)[NS], Store st) {
+                                              double (&y1)[NS], double (&err)[NS], Store st,
+                                              lds_double *kl = nullptr) {
   using R = R5;
+  const int lane = lane_id();
   auto eval_rhs_ = [&](double tau, const double (&y)[NS], double (&f)[NS]) {
     if constexpr (CALLS) rhs_call<NS>(cx, m.rd, tau, y, f);
     else eval_rhs<NS, double>(cx, m.rd, tau, y, f);
@@ -390,81 +395,100 @@ __device__ __forceinline__ void rodas5_step_s(Mode<NS> &m, Ctx &cx, double t0, d
   else factor<NS>(m, cx, t0, y0, dt);
   double dT[NS];
   dfdtau<NS>(m.rd, t0, y0, dT);
-  double k1[NS], k2[NS], k3[NS], k4[NS], k5[NS], k6[NS], k7[NS];
-  double u[NS], f[NS];
+  double kr[7][NS];
+  // stage vector i (0-based) of slot s: registers, or LDS when kl is given
+  auto K = [&](int i, int s) -> double { return kl ? (double)kl[(i * NS + s) * WAVE + lane] : kr[i][s]; };
+  auto keep = [&](int i, const double (&k)[NS]) {
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      if (kl) kl[(i * NS + s) * WAVE + lane] = k[s];
+      else kr[i][s] = k[s];
+    }
+  };
+  double k[NS], u[NS], f[NS];
   const double idt = 1.0 / dt;
   // stage 1
   eval_rhs_(t0, y0, f);
 #pragma unroll
-  for (int s = 0; s < NS; ++s) k1[s] = f[s] + dt * R::d1 * dT[s];
-  wsolve_(k1);
-  st(0, k1);
+  for (int s = 0; s < NS; ++s) k[s] = f[s] + dt * R::d1 * dT[s];
+  wsolve_(k);
+  st(0, k);
+  keep(0, k);
   // stage 2
 #pragma unroll
-  for (int s = 0; s < NS; ++s) u[s] = y0[s] + R::a21 * k1[s];
+  for (int s = 0; s < NS; ++s) u[s] = y0[s] + R::a21 * K(0, s);
   eval_rhs_(t0 + R::c2 * dt, u, f);
 #pragma unroll
-  for (int s = 0; s < NS; ++s) k2[s] = f[s] + dt * R::d2 * dT[s] + (R::C21 * idt) * k1[s];
-  wsolve_(k2);
-  st(1, k2);
+  for (int s = 0; s < NS; ++s) k[s] = f[s] + dt * R::d2 * dT[s] + (R::C21 * idt) * K(0, s);
+  wsolve_(k);
+  st(1, k);
+  keep(1, k);
   // stage 3
 #pragma unroll
-  for (int s = 0; s < NS; ++s) u[s] = y0[s] + R::a31 * k1[s] + R::a32 * k2[s];
+  for (int s = 0; s < NS; ++s) u[s] = y0[s] + R::a31 * K(0, s) + R::a32 * K(1, s);
   eval_rhs_(t0 + R::c3 * dt, u, f);
 #pragma unroll
   for (int s = 0; s < NS; ++s)
-    k3[s] = f[s] + dt * R::d3 * dT[s] + ((R::C31 * idt) * k1[s] + (R::C32 * idt) * k2[s]);
-  wsolve_(k3);
-  st(2, k3);
+    k[s] = f[s] + dt * R::d3 * dT[s] + ((R::C31 * idt) * K(0, s) + (R::C32 * idt) * K(1, s));
+  wsolve_(k);
+  st(2, k);
+  keep(2, k);
   // stage 4
 #pragma unroll
-  for (int s = 0; s < NS; ++s) u[s] = y0[s] + R::a41 * k1[s] + R::a42 * k2[s] + R::a43 * k3[s];
+  for (int s = 0; s < NS; ++s) u[s] = y0[s] + R::a41 * K(0, s) + R::a42 * K(1, s) + R::a43 * K(2, s);
   eval_rhs_(t0 + R::c4 * dt, u, f);
 #pragma unroll
   for (int s = 0; s < NS; ++s)
-    k4[s] = f[s] + dt * R::d4 * dT[s] + ((R::C41 * idt) * k1[s] + (R::C42 * idt) * k2[s] + (R::C43 * idt) * k3[s]);
-  wsolve_(k4);
-  st(3, k4);
+    k[s] = f[s] + dt * R::d4 * dT[s] +
+           ((R::C41 * idt) * K(0, s) + (R::C42 * idt) * K(1, s) + (R::C43 * idt) * K(2, s));
+  wsolve_(k);
+  st(3, k);
+  keep(3, k);
   // stage 5
 #pragma unroll
-  for (int s = 0; s < NS; ++s) u[s] = y0[s] + R::a51 * k1[s] + R::a52 * k2[s] + R::a53 * k3[s] + R::a54 * k4[s];
+  for (int s = 0; s < NS; ++s)
+    u[s] = y0[s] + R::a51 * K(0, s) + R::a52 * K(1, s) + R::a53 * K(2, s) + R::a54 * K(3, s);
   eval_rhs_(t0 + R::c5 * dt, u, f);
 #pragma unroll
   for (int s = 0; s < NS; ++s)
-    k5[s] = f[s] + dt * R::d5 * dT[s] +
-            ((R::C51 * idt) * k1[s] + (R::C52 * idt) * k2[s] + (R::C53 * idt) * k3[s] + (R::C54 * idt) * k4[s]);
-  wsolve_(k5);
-  st(4, k5);
+    k[s] = f[s] + dt * R::d5 * dT[s] +
+           ((R::C51 * idt) * K(0, s) + (R::C52 * idt) * K(1, s) + (R::C53 * idt) * K(2, s) +
+            (R::C54 * idt) * K(3, s));
+  wsolve_(k);
+  st(4, k);
+  keep(4, k);
   // stage 6
 #pragma unroll
   for (int s = 0; s < NS; ++s)
-    u[s] = y0[s] + R::a61 * k1[s] + R::a62 * k2[s] + R::a63 * k3[s] + R::a64 * k4[s] + R::a65 * k5[s];
+    u[s] = y0[s] + R::a61 * K(0, s) + R::a62 * K(1, s) + R::a63 * K(2, s) + R::a64 * K(3, s) + R::a65 * K(4, s);
   eval_rhs_(t0 + dt, u, f);
 #pragma unroll
   for (int s = 0; s < NS; ++s)
-    k6[s] = f[s] + ((R::C61 * idt) * k1[s] + (R::C62 * idt) * k2[s] + (R::C63 * idt) * k3[s] +
-                    (R::C64 * idt) * k4[s] + (R::C65 * idt) * k5[s]);
-  wsolve_(k6);
-  st(5, k6);
+    k[s] = f[s] + ((R::C61 * idt) * K(0, s) + (R::C62 * idt) * K(1, s) + (R::C63 * idt) * K(2, s) +
+                   (R::C64 * idt) * K(3, s) + (R::C65 * idt) * K(4, s));
+  wsolve_(k);
+  st(5, k);
+  keep(5, k);
   // stage 7
 #pragma unroll
-  for (int s = 0; s < NS; ++s) u[s] = u[s] + k6[s];
+  for (int s = 0; s < NS; ++s) u[s] = u[s] + k[s];
   eval_rhs_(t0 + dt, u, f);
 #pragma unroll
   for (int s = 0; s < NS; ++s)
-    k7[s] = f[s] + ((R::C71 * idt) * k1[s] + (R::C72 * idt) * k2[s] + (R::C73 * idt) * k3[s] +
-                    (R::C74 * idt) * k4[s] + (R::C75 * idt) * k5[s] + (R::C76 * idt) * k6[s]);
-  wsolve_(k7);
-  st(6, k7);
+    k[s] = f[s] + ((R::C71 * idt) * K(0, s) + (R::C72 * idt) * K(1, s) + (R::C73 * idt) * K(2, s) +
+                   (R::C74 * idt) * K(3, s) + (R::C75 * idt) * K(4, s) + (R::C76 * idt) * K(5, s));
+  wsolve_(k);
+  st(6, k);
+  keep(6, k);
   // stage 8
 #pragma unroll
-  for (int s = 0; s < NS; ++s) u[s] = u[s] + k7[s];
+  for (int s = 0; s < NS; ++s) u[s] = u[s] + k[s];
   eval_rhs_(t0 + dt, u, f);
 #pragma unroll
   for (int s = 0; s < NS; ++s)
-    err[s] = f[s] + ((R::C81 * idt) * k1[s] + (R::C82 * idt) * k2[s] + (R::C83 * idt) * k3[s] +
-                     (R::C84 * idt) * k4[s] + (R::C85 * idt) * k5[s] + (R::C86 * idt) * k6[s] +
-                     (R::C87 * idt) * k7[s]);
+    err[s] = f[s] + ((R::C81 * idt) * K(0, s) + (R::C82 * idt) * K(1, s) + (R::C83 * idt) * K(2, s) +
+                     (R::C84 * idt) * K(3, s) + (R::C85 * idt) * K(4, s) + (R::C86 * idt) * K(5, s) +
+                     (R::C87 * idt) * K(6, s));
   wsolve_(err);
   st(7, err);
 #pragma unroll
@@ -473,8 +497,8 @@ __device__ __forceinline__ void rodas5_step_s(Mode<NS> &m, Ctx &cx, double t0, d
 
 template <int NS>
 __device__ __forceinline__ void rodas5_step(Mode<NS> &m, Ctx &cx, double t0, double dt, const double (&y0)[NS],
-                                            double (&y1)[NS], double (&err)[NS]) {
-  rodas5_step_s<NS>(m, cx, t0, dt, y0, y1, err, [](int, const double (&)[NS]) {});
+                                            double (&y1)[NS], double (&err)[NS], lds_double *kl = nullptr) {
+  rodas5_step_s<NS>(m, cx, t0, dt, y0, y1, err, [](int, const double (&)[NS]) {}, kl);
 }
 
 }  // namespace deb

@@ -58,8 +58,8 @@ def lib():
         L.deb_solve_forward_tape.argtypes = [ctypes.POINTER(DebConfig), ctypes.c_int, _dp, _dp, _i64, _dp, _dp, _dp,
                                              ctypes.c_int, _dp, ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_int, _dp,
                                              ctypes.c_size_t, _dp]
-        L.deb_adjoint_workspace_bytes.argtypes = [ctypes.POINTER(DebConfig), ctypes.c_int,
-                                                  ctypes.POINTER(ctypes.c_size_t)]
+        L.deb_adjoint_workspace_bytes.argtypes = [ctypes.POINTER(DebConfig), ctypes.c_int, ctypes.c_int,
+                                                  ctypes.c_int, ctypes.POINTER(ctypes.c_size_t)]
         L.deb_solve_adjoint.argtypes = [ctypes.POINTER(DebConfig), ctypes.c_int, _dp, _dp, _i64, _dp, _dp, _dp,
                                         ctypes.c_int, _dp, ctypes.c_int, _dp, ctypes.c_int, _dp, _dp, _dp, _dp, _dp,
                                         _dp, ctypes.c_int, _dp, ctypes.c_size_t, _dp, ctypes.c_size_t, _dp]
@@ -78,7 +78,7 @@ def lib():
                                        ctypes.c_size_t, _dp, _dp]
         L.deb_power_multipoles.argtypes = [_i, _i, _dp, _dp, _i, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
         L.deb_xi_from_power.argtypes = [_i, _i, _dp, _dp, _i, _dp, _dp, _dp]
-        if L.deb_abi_version() != 2:
+        if L.deb_abi_version() != 3:
             raise DebError("libdiscoeb_hip.so ABI version mismatch")
         _LIB = L
     return _LIB

@@ -46,10 +46,10 @@ def tape_bytes(cfg, n_cosmo: int, nk: int, n_states: int, ckpt_every: int = CKPT
     return int(b.value)
 
 
-def adjoint_workspace_bytes(cfg, ckpt_every: int = CKPT_EVERY) -> int:
+def adjoint_workspace_bytes(cfg, n_cosmo: int, nk: int, ckpt_every: int = CKPT_EVERY) -> int:
     b = ctypes.c_size_t(0)
-    _lib.check(_lib.lib().deb_adjoint_workspace_bytes(ctypes.byref(cfg), int(ckpt_every), ctypes.byref(b)),
-               "deb_adjoint_workspace_bytes")
+    _lib.check(_lib.lib().deb_adjoint_workspace_bytes(ctypes.byref(cfg), int(n_cosmo), int(nk), int(ckpt_every),
+                                                      ctypes.byref(b)), "deb_adjoint_workspace_bytes")
     return int(b.value)
 
 
@@ -70,7 +70,7 @@ def solve_forward_tape(inp: DeviceInputs, kmodes, tau_out, *, n_states: int | No
     st = stream if stream is not None else torch.cuda.current_stream(inp.device)
     if n_states is None:
         n_states = n_cosmo * nk * STATES_PER_MODE
-    ws = adjoint_workspace_bytes(cfg, ckpt_every)
+    ws = adjoint_workspace_bytes(cfg, n_cosmo, nk, ckpt_every)
     for attempt in range(2):
         nb = tape_bytes(cfg, n_cosmo, nk, n_states, ckpt_every)
         if nb + ws > budget_bytes:
@@ -117,7 +117,7 @@ def solve_adjoint(inp: DeviceInputs, tan: Tangents, kmodes, tau_out, ybar, statu
     n_cosmo, nk = kmodes.shape
     nout = tau_out.shape[1]
     grad = torch.empty((n_cosmo, nk, tan.ndir), dtype=torch.float64, device=inp.device)
-    wsb = adjoint_workspace_bytes(cfg, tape.ckpt_every)
+    wsb = adjoint_workspace_bytes(cfg, n_cosmo, nk, tape.ckpt_every)
     ws = torch.empty(wsb, dtype=torch.uint8, device=inp.device)
     st = stream if stream is not None else torch.cuda.current_stream(inp.device)
     _lib.check(L.deb_solve_adjoint(ctypes.byref(cfg), n_cosmo, _lib.ptr(inp.cosmo), _lib.ptr(inp.tables), inp.stride,

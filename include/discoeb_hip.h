@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define DEB_ABI_VERSION 2
+#define DEB_ABI_VERSION 3
 #define DEB_MAXNQ 16    /* maximum massive-neutrino momentum bins */
 #define DEB_NOUT_FIELDS 20
 #define DEB_MAXDIR 6    /* input tangent directions per gradient launch */
@@ -248,12 +248,14 @@ int deb_solve_forward_tape(const deb_config_t *cfg, int n_cosmo, const deb_cosmo
                            int32_t *nsteps, int ckpt_every, void *tape, size_t tape_bytes, void *stream);
 
 /*
- * Bytes of device workspace deb_solve_adjoint needs: one ring per wave the
- * current device can hold at the reverse kernel's occupancy, times two (each
- * ring holds a segment's ckpt_every + 1 states and 8 ckpt_every stage vectors,
- * 512 NS (9 ckpt_every + 1) bytes).  Queries the current HIP device.
+ * Bytes of device workspace deb_solve_adjoint needs for n_cosmo x nk modes:
+ * one ring per wave the current device can hold at the reverse kernel's
+ * occupancy, times two (each ring holds a segment's ckpt_every + 1 states and
+ * 8 ckpt_every stage vectors, 512 NS (9 ckpt_every + 1) bytes), plus the
+ * launch order of the modes (longest first, from the tape's step counts; 4
+ * bytes per mode and a 32 KB histogram).  Queries the current HIP device.
  */
-int deb_adjoint_workspace_bytes(const deb_config_t *cfg, int ckpt_every, size_t *bytes);
+int deb_adjoint_workspace_bytes(const deb_config_t *cfg, int n_cosmo, int nk, int ckpt_every, size_t *bytes);
 
 /*
  * Reverse sweep over a tape written by deb_solve_forward_tape with the same

@@ -88,7 +88,7 @@ int run(const deb_config_t &cfg, const Layout &L, Reader &in, FILE *out, int mod
   P.nsteps = nsteps.data();
   if (mode == 0) {
     for (int ki = 0; ki < nk; ++ki) {
-      std::vector<double> lds(NS * WAVE, -777.0), pcr(pcr_lds_doubles<NS>(), -444.0);
+      std::vector<double> lds(fwd_lds_doubles<NS>(), -777.0), pcr(pcr_lds_doubles<NS>(), -444.0);
       run_wave([&](int) { integrate_mode<NS>(P, 0, ki, lds.data(), pcr.data()); });
     }
     fwrite(status.data(), 4, nk, out);
@@ -111,7 +111,7 @@ int run(const deb_config_t &cfg, const Layout &L, Reader &in, FILE *out, int mod
   TapeView tv;
   if (int r = tape_view(tv, tape.data(), tape_bytes, nk, NS, ckpt)) return r;
   for (int ki = 0; ki < nk; ++ki) {
-    std::vector<double> lds(NS * WAVE, -777.0), pcr(pcr_lds_doubles<NS>(), -444.0);
+    std::vector<double> lds(fwd_lds_doubles<NS>(), -777.0), pcr(pcr_lds_doubles<NS>(), -444.0);
     run_wave([&](int) { integrate_mode<NS, true>(P, 0, ki, lds.data(), pcr.data(), &tv); });
   }
   AdjArgs A;

@@ -35,7 +35,7 @@ def test_host_argument_checks_without_gpu():
     if not os.path.exists(_lib.LIB_PATH):
         pytest.skip("libdiscoeb_hip.so not built")
     L = _lib.lib()
-    assert L.deb_abi_version() == 2
+    assert L.deb_abi_version() == 3
     cfg = packing.make_config(lmaxg=32, lmaxgp=32, lmaxr=32, lmaxnu=17, nqmax=5)
     assert L.deb_nvar(ctypes.byref(cfg)) == packing.nvar(32, 32, 32, 17, 5) == 198
     b = ctypes.c_size_t(0)

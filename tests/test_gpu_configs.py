@@ -310,3 +310,4 @@ def test_c3_full_size_launch(gpu):
         inp1 = DeviceInputs(sub, cfg)
         y1, st1, _ = solve_forward(inp1, torch.from_numpy(k[None]), torch.from_numpy(tau[b:b + 1]), throw=False)
         assert torch.equal(y[b], y1[0]) and torch.equal(st[b], st1[0])
+
