@@ -355,216 +355,159 @@ DEB_OUTLINE RowSums coef_pass(Ctx &cx, const uint32_t (&rd)[NS], double a, doubl
   return rs;
 }
 
-// partials of one row's sum_fields mul[q][f] coef_f with respect to the
-// features (accumulated into fb[q]) and to its bin velocity (vb[q]), for K
-// contractions at once (one pass of the divergent row switch); fv[q]: the
-// feature values of contraction q
+// Partials of the rows' sum_fields mul[q][f] coef_f with respect to the
+// features (accumulated into fb[q]) and to their bin velocity (vb[q]), for K
+// contractions at once; fv[q]: the feature values of contraction q.  The
+// partials of row_visit's formulas, organised for the wave rather than as one
+// divergent switch over row types:
+//   chain_feature_adj    the rows of the hierarchies (any slot, any lane):
+//                        per-lane arithmetic with selects, no branches;
+//   special_feature_adj  the rows that sit on fixed lanes of slot 0 (eta,
+//                        delta_c, theta_c, delta_b, theta_b, delta_gamma,
+//                        theta_gamma, delta_Q, theta_Q): their multipliers are
+//                        read from those lanes and their partials formed as
+//                        wave-uniform values, to be added once (on lane 0).
 template <int K, class T = double>
-__device__ __forceinline__ void row_feature_adj(uint32_t d, const Ctx &cx, const T (&fv)[K][FT_N],
-                                                const T (&vq)[K], const T (&mul)[K][11], T (&fb)[K][FT_N],
-                                                T (&vb)[K]) {
+__device__ __forceinline__ void chain_feature_adj(uint32_t d, const Ctx &cx, const T (&fv)[K][FT_N],
+                                                  const T (&vq)[K], const T (&mul)[K][11], T (&fb)[K][FT_N],
+                                                  T (&vb)[K]) {
   const double k = cx.k, k2 = k * k;
-  const double ik2 = 1.0 / k2, hk2 = 0.5 * ik2;  // wave-uniform
+  const double hk2 = 0.5 / k2;
   const uint32_t t = rd_type(d);
   const double l = (double)rd_l(d);
-  const int qi = (int)rd_q(d);
-  switch (t) {
-    case R_ETA:
+  // opacity through the diagonal (-opac) and the polarization source (c opac)
+  const bool od = t == R_F2 || t == R_FL || t == R_FLMAX || t == R_G0 || t == R_GL || t == R_G2 || t == R_GLMAX;
+  const double cu = (t == R_F2 || t == R_G2) ? 0.1 : (t == R_G0 ? 0.5 : 0.0);
+  // massive-nu rows: sub, sup proportional to k v_q
+  const bool qr = t >= R_Q0 && t <= R_QLMAX;
+  const double il = rcp_nr(2.0 * l + 1.0);
+  const double cs = t == R_Q1 ? 1.0 / 3.0 : (t == R_Q2 ? 0.4 : (t == R_QL ? l * il : (t == R_QLMAX ? 1.0 : 0.0)));
+  const double cp = t == R_Q0 ? 1.0 : (t == R_Q1 ? 2.0 / 3.0 : (t == R_Q2 ? 0.6 : (t == R_QL ? (l + 1.0) * il : 0.0)));
+  const bool q0 = t == R_Q0, q1 = t == R_Q1, dr = t == R_DR, tr = t == R_TR;
+  const double wq = (q0 || q1) ? cx.w[rd_q(d)] : 0.0;
+  const T zero = tconst<T>(0.0);
 #pragma unroll
-      for (int q = 0; q < K; ++q) {
-        fb[q][FT_IAH] += mul[q][C_CH] * 2.0 * k2;
-      }
-      break;
-    case R_DC:
-#pragma unroll
-      for (int q = 0; q < K; ++q) {
-        fb[q][FT_RC] += mul[q][C_CH] * fv[q][FT_IAH] - mul[q][C_C1] * fv[q][FT_A];
-        fb[q][FT_IAH] += mul[q][C_CH] * fv[q][FT_RC];
-        fb[q][FT_A] += -mul[q][C_C1] * fv[q][FT_RC];
-      }
-      break;
-    case R_DB:
-#pragma unroll
-      for (int q = 0; q < K; ++q) {
-        fb[q][FT_RB] += mul[q][C_CH] * fv[q][FT_IAH] - mul[q][C_C1] * fv[q][FT_A];
-        fb[q][FT_IAH] += mul[q][C_CH] * fv[q][FT_RB];
-        fb[q][FT_A] += -mul[q][C_C1] * fv[q][FT_RB];
-      }
-      break;
-    case R_TC:
-#pragma unroll
-      for (int q = 0; q < K; ++q) {
-        fb[q][FT_AH] += -mul[q][C_DIAG];
-        fb[q][FT_RC] += mul[q][C_CETA] * hk2;
-      }
-      break;
-    case R_TB:
-#pragma unroll
-      for (int q = 0; q < K; ++q) {
-        fb[q][FT_CS2] += mul[q][C_SUB] * k2;
-        fb[q][FT_AH] += -mul[q][C_DIAG];
-        fb[q][FT_PB43] += (mul[q][C_SUP] - mul[q][C_DIAG]) * fv[q][FT_OPAC];
-        fb[q][FT_OPAC] += (mul[q][C_SUP] - mul[q][C_DIAG]) * fv[q][FT_PB43];
-        fb[q][FT_RB] += mul[q][C_CETA] * hk2;
-      }
-      break;
-    case R_DG:
-#pragma unroll
-      for (int q = 0; q < K; ++q) {
-        fb[q][FT_RG] += mul[q][C_CH] * fv[q][FT_IAH] - 2.0 * mul[q][C_C1] * fv[q][FT_A];
-        fb[q][FT_IAH] += mul[q][C_CH] * fv[q][FT_RG];
-        fb[q][FT_A] += -2.0 * mul[q][C_C1] * fv[q][FT_RG];
-      }
-      break;
-    case R_TG:
-#pragma unroll
-      for (int q = 0; q < K; ++q) {
-        fb[q][FT_OPAC] += mul[q][C_SUB] - mul[q][C_DIAG];
-        fb[q][FT_RG] += mul[q][C_CETA] * (4.0 / 3.0) * hk2;
-      }
-      break;
-    case R_F2:
-#pragma unroll
-      for (int q = 0; q < K; ++q) {
-        fb[q][FT_OPAC] += -mul[q][C_DIAG] + 0.1 * mul[q][C_UPI];
-      }
-      break;
-    case R_G0:
-#pragma unroll
-      for (int q = 0; q < K; ++q) {
-        fb[q][FT_OPAC] += -mul[q][C_DIAG] + 0.5 * mul[q][C_UPI];
-      }
-      break;
-    case R_G2:
-#pragma unroll
-      for (int q = 0; q < K; ++q) {
-        fb[q][FT_OPAC] += -mul[q][C_DIAG] + 0.1 * mul[q][C_UPI];
-      }
-      break;
-    case R_FL:
-    case R_FLMAX:
-    case R_GL:
-    case R_GLMAX:
-#pragma unroll
-      for (int q = 0; q < K; ++q) {
-        fb[q][FT_OPAC] += -mul[q][C_DIAG];
-      }
-      break;
-    case R_DR:
-#pragma unroll
-      for (int q = 0; q < K; ++q) {
-        fb[q][FT_RR] += mul[q][C_CH] * fv[q][FT_IAH] - 2.0 * mul[q][C_C1] * fv[q][FT_A];
-        fb[q][FT_IAH] += mul[q][C_CH] * fv[q][FT_RR];
-        fb[q][FT_A] += -2.0 * mul[q][C_C1] * fv[q][FT_RR];
-      }
-      break;
-    case R_TR:
-#pragma unroll
-      for (int q = 0; q < K; ++q) {
-        fb[q][FT_RR] += mul[q][C_CETA] * (4.0 / 3.0) * hk2;
-      }
-      break;
-    case R_Q0: {
-#pragma unroll
-      for (int q = 0; q < K; ++q) {
-        const double wq = cx.w[qi];
-        const T RN = fv[q][FT_RN];
-        const T iv = rcp_t<T>(vq[q]);
-        vb[q] += -k * mul[q][C_SUP];
-        // ch = RN wq / v * iaH
-        fb[q][FT_RN] += mul[q][C_CH] * wq * iv * fv[q][FT_IAH];
-        fb[q][FT_IAH] += mul[q][C_CH] * RN * wq * iv;
-        vb[q] += -mul[q][C_CH] * RN * wq * fv[q][FT_IAH] * iv * iv;
-        // c1 = -a (RN wq / v + RN wq v)
-        fb[q][FT_A] += -mul[q][C_C1] * RN * wq * (iv + vq[q]);
-        fb[q][FT_RN] += -mul[q][C_C1] * fv[q][FT_A] * wq * (iv + vq[q]);
-        vb[q] += -mul[q][C_C1] * fv[q][FT_A] * RN * wq * (1.0 - iv * iv);
-      }
-      break;
+  for (int q = 0; q < K; ++q) {
+    const T *m = mul[q];
+    const T *F = fv[q];
+    if (od) fb[q][FT_OPAC] += -m[C_DIAG] + (cu != 0.0 ? cu * m[C_UPI] : zero);
+    if (qr) vb[q] += k * (m[C_SUB] * cs - m[C_SUP] * cp);
+    if (q0) {
+      // ch = RN wq / v * iaH; c1 = -a (RN wq / v + RN wq v)
+      const T RN = F[FT_RN];
+      const T iv = rcp_t<T>(vq[q]);
+      fb[q][FT_RN] += m[C_CH] * wq * iv * F[FT_IAH] - m[C_C1] * F[FT_A] * wq * (iv + vq[q]);
+      fb[q][FT_IAH] += m[C_CH] * RN * wq * iv;
+      vb[q] += -m[C_CH] * RN * wq * F[FT_IAH] * iv * iv - m[C_C1] * F[FT_A] * RN * wq * (1.0 - iv * iv);
+      fb[q][FT_A] += -m[C_C1] * RN * wq * (iv + vq[q]);
     }
-    case R_Q1:
-#pragma unroll
-      for (int q = 0; q < K; ++q) {
-        vb[q] += k * (mul[q][C_SUB] * (1.0 / 3.0) - mul[q][C_SUP] * (2.0 / 3.0));
-        fb[q][FT_RN] += mul[q][C_CETA] * k * cx.w[qi] * hk2;
-      }
-      break;
-    case R_Q2:
-#pragma unroll
-      for (int q = 0; q < K; ++q) {
-        vb[q] += k * (0.4 * mul[q][C_SUB] - 0.6 * mul[q][C_SUP]);
-      }
-      break;
-    case R_QL: {
-      const double il = rcp_nr(2.0 * l + 1.0);
-#pragma unroll
-      for (int q = 0; q < K; ++q) {
-        vb[q] += k * (mul[q][C_SUB] * (l * il) - mul[q][C_SUP] * ((l + 1.0) * il));
-      }
-      break;
+    if (q1) fb[q][FT_RN] += m[C_CETA] * k * wq * hk2;
+    if (dr) {
+      fb[q][FT_RR] += m[C_CH] * F[FT_IAH] - 2.0 * m[C_C1] * F[FT_A];
+      fb[q][FT_IAH] += m[C_CH] * F[FT_RR];
+      fb[q][FT_A] += -2.0 * m[C_C1] * F[FT_RR];
     }
-    case R_QLMAX:
+    if (tr) fb[q][FT_RR] += m[C_CETA] * (4.0 / 3.0) * hk2;
+  }
+}
+
+// the fixed-lane rows of slot 0 (multipliers ms0 of slot 0): wave-uniform
+// partials added into sp
+template <int K, class T = double>
+__device__ __forceinline__ void special_feature_adj(const Ctx &cx, const T (&fv)[K][FT_N], const T (&ms0)[K][11],
+                                                    T (&sp)[K][FT_N]) {
+  const double k = cx.k, k2 = k * k;
+  const double ik2 = 1.0 / k2, hk2 = 0.5 * ik2;
 #pragma unroll
-      for (int q = 0; q < K; ++q) {
-        vb[q] += k * mul[q][C_SUB];
+  for (int q = 0; q < K; ++q) {
+    const T *F = fv[q];
+    auto M = [&](int lane, int f) { return t_readlane(ms0[q][f], lane); };
+    T *o = sp[q];
+    // eta: ch = 2 k^2 iaH
+    o[FT_IAH] += M(L_ETA, C_CH) * 2.0 * k2;
+    // delta_c, delta_b: ch = R iaH, c1 = -a R (R = rho_c / a, rho_b / a)
+    {
+      const T ch = M(L_DC, C_CH), c1 = M(L_DC, C_C1);
+      o[FT_RC] += ch * F[FT_IAH] - c1 * F[FT_A];
+      o[FT_IAH] += ch * F[FT_RC];
+      o[FT_A] += -c1 * F[FT_RC];
+    }
+    {
+      const T ch = M(L_DB, C_CH), c1 = M(L_DB, C_C1);
+      o[FT_RB] += ch * F[FT_IAH] - c1 * F[FT_A];
+      o[FT_IAH] += ch * F[FT_RB];
+      o[FT_A] += -c1 * F[FT_RB];
+    }
+    // theta_c: diag = -aH, ceta = R_c hk2
+    o[FT_AH] += -M(L_TC, C_DIAG);
+    o[FT_RC] += M(L_TC, C_CETA) * hk2;
+    // theta_b: sub = k^2 cs2, diag = -aH - pb43 opac, sup = pb43 opac, ceta = R_b hk2
+    {
+      const T sub = M(L_TB, C_SUB), dg = M(L_TB, C_DIAG), sup = M(L_TB, C_SUP);
+      o[FT_CS2] += sub * k2;
+      o[FT_AH] += -dg;
+      o[FT_PB43] += (sup - dg) * F[FT_OPAC];
+      o[FT_OPAC] += (sup - dg) * F[FT_PB43];
+      o[FT_RB] += M(L_TB, C_CETA) * hk2;
+    }
+    // delta_gamma: ch = R_g iaH, c1 = -2 a R_g
+    {
+      const T ch = M(L_DG, C_CH), c1 = M(L_DG, C_C1);
+      o[FT_RG] += ch * F[FT_IAH] - 2.0 * c1 * F[FT_A];
+      o[FT_IAH] += ch * F[FT_RG];
+      o[FT_A] += -2.0 * c1 * F[FT_RG];
+    }
+    // theta_gamma: sub = opac, diag = -opac, ceta = (4/3) R_g hk2
+    o[FT_OPAC] += M(L_TG, C_SUB) - M(L_TG, C_DIAG);
+    o[FT_RG] += M(L_TG, C_CETA) * (4.0 / 3.0) * hk2;
+    if (!cx.lambda) {
+      const T w = F[FT_W], ca2 = F[FT_CA2], cs2Q = F[FT_CS2Q], RQ = F[FT_RQ];
+      const T opw = 1.0 + w;
+      {  // delta_Q
+        const T dg = M(L_DQ, C_DIAG), sup = M(L_DQ, C_SUP), uh = M(L_DQ, C_UH), ch = M(L_DQ, C_CH),
+                c1 = M(L_DQ, C_C1);
+        const T g = F[FT_AH] * F[FT_AH] * ik2;
+        // diag = -3 (cs2Q - w) aH
+        o[FT_CS2Q] += -3.0 * F[FT_AH] * dg;
+        o[FT_W] += 3.0 * F[FT_AH] * dg;
+        o[FT_AH] += -3.0 * (cs2Q - w) * dg;
+        // sup = -opw - 9 opw (cs2Q - ca2) aH^2 / k2
+        o[FT_W] += sup * (-1.0 - 9.0 * (cs2Q - ca2) * g);
+        o[FT_CS2Q] += -9.0 * opw * g * sup;
+        o[FT_CA2] += 9.0 * opw * g * sup;
+        o[FT_AH] += -18.0 * opw * (cs2Q - ca2) * F[FT_AH] * ik2 * sup;
+        // uh = -0.5 opw
+        o[FT_W] += -0.5 * uh;
+        // ch = RQ iaH
+        o[FT_RQ] += ch * F[FT_IAH];
+        o[FT_IAH] += ch * RQ;
+        // c1 = -a (RQ + 3 cs2Q RQ)
+        o[FT_A] += -c1 * RQ * (1.0 + 3.0 * cs2Q);
+        o[FT_RQ] += -c1 * F[FT_A] * (1.0 + 3.0 * cs2Q);
+        o[FT_CS2Q] += -c1 * 3.0 * F[FT_A] * RQ;
       }
-      break;
-    case R_DQ:
-#pragma unroll
-      for (int q = 0; q < K; ++q) {
-        if (!cx.lambda) {
-          const T w = fv[q][FT_W], ca2 = fv[q][FT_CA2], cs2Q = fv[q][FT_CS2Q], RQ = fv[q][FT_RQ];
-          const T opw = 1.0 + w, g = fv[q][FT_AH] * fv[q][FT_AH] * ik2;
-          // diag = -3 (cs2Q - w) aH
-          fb[q][FT_CS2Q] += -3.0 * fv[q][FT_AH] * mul[q][C_DIAG];
-          fb[q][FT_W] += 3.0 * fv[q][FT_AH] * mul[q][C_DIAG];
-          fb[q][FT_AH] += -3.0 * (cs2Q - w) * mul[q][C_DIAG];
-          // sup = -opw - 9 opw (cs2Q - ca2) aH^2 / k2
-          fb[q][FT_W] += mul[q][C_SUP] * (-1.0 - 9.0 * (cs2Q - ca2) * g);
-          fb[q][FT_CS2Q] += -9.0 * opw * g * mul[q][C_SUP];
-          fb[q][FT_CA2] += 9.0 * opw * g * mul[q][C_SUP];
-          fb[q][FT_AH] += -18.0 * opw * (cs2Q - ca2) * fv[q][FT_AH] * ik2 * mul[q][C_SUP];
-          // uh = -0.5 opw
-          fb[q][FT_W] += -0.5 * mul[q][C_UH];
-          // ch = RQ iaH
-          fb[q][FT_RQ] += mul[q][C_CH] * fv[q][FT_IAH];
-          fb[q][FT_IAH] += mul[q][C_CH] * RQ;
-          // c1 = -a (RQ + 3 cs2Q RQ)
-          fb[q][FT_A] += -mul[q][C_C1] * RQ * (1.0 + 3.0 * cs2Q);
-          fb[q][FT_RQ] += -mul[q][C_C1] * fv[q][FT_A] * (1.0 + 3.0 * cs2Q);
-          fb[q][FT_CS2Q] += -mul[q][C_C1] * 3.0 * fv[q][FT_A] * RQ;
-        }
+      {  // theta_Q
+        const T sub = M(L_TQ, C_SUB), dg = M(L_TQ, C_DIAG), ce = M(L_TQ, C_CETA), c1 = M(L_TQ, C_C1);
+        // sub = cs2Q / opw k2
+        const T iopw = rcp_t<T>(opw);
+        o[FT_CS2Q] += sub * k2 * iopw;
+        o[FT_W] += -sub * cs2Q * k2 * (iopw * iopw);
+        // diag = -(1 - 3 cs2Q) aH
+        o[FT_CS2Q] += 3.0 * F[FT_AH] * dg;
+        o[FT_AH] += -(1.0 - 3.0 * cs2Q) * dg;
+        // ceta = opw RQ 0.5 / k2
+        o[FT_W] += ce * RQ * hk2;
+        o[FT_RQ] += ce * opw * hk2;
+        // c1 = -a 9 (cs2Q - ca2) aH opw RQ / k2
+        const double c9 = 9.0 * ik2;
+        o[FT_A] += -c1 * c9 * (cs2Q - ca2) * F[FT_AH] * opw * RQ;
+        o[FT_CS2Q] += -c1 * F[FT_A] * c9 * F[FT_AH] * opw * RQ;
+        o[FT_CA2] += c1 * F[FT_A] * c9 * F[FT_AH] * opw * RQ;
+        o[FT_AH] += -c1 * F[FT_A] * c9 * (cs2Q - ca2) * opw * RQ;
+        o[FT_W] += -c1 * F[FT_A] * c9 * (cs2Q - ca2) * F[FT_AH] * RQ;
+        o[FT_RQ] += -c1 * F[FT_A] * c9 * (cs2Q - ca2) * F[FT_AH] * opw;
       }
-      break;
-    case R_TQ:
-#pragma unroll
-      for (int q = 0; q < K; ++q) {
-        if (!cx.lambda) {
-          const T w = fv[q][FT_W], ca2 = fv[q][FT_CA2], cs2Q = fv[q][FT_CS2Q], RQ = fv[q][FT_RQ];
-          const T opw = 1.0 + w;
-          // sub = cs2Q / opw k2
-          const T iopw = rcp_t<T>(opw);
-          fb[q][FT_CS2Q] += mul[q][C_SUB] * k2 * iopw;
-          fb[q][FT_W] += -mul[q][C_SUB] * cs2Q * k2 * (iopw * iopw);
-          // diag = -(1 - 3 cs2Q) aH
-          fb[q][FT_CS2Q] += 3.0 * fv[q][FT_AH] * mul[q][C_DIAG];
-          fb[q][FT_AH] += -(1.0 - 3.0 * cs2Q) * mul[q][C_DIAG];
-          // ceta = opw RQ 0.5 / k2
-          fb[q][FT_W] += mul[q][C_CETA] * RQ * hk2;
-          fb[q][FT_RQ] += mul[q][C_CETA] * opw * hk2;
-          // c1 = -a 9 (cs2Q - ca2) aH opw RQ / k2
-          const double c9 = 9.0 * ik2;
-          fb[q][FT_A] += -mul[q][C_C1] * c9 * (cs2Q - ca2) * fv[q][FT_AH] * opw * RQ;
-          fb[q][FT_CS2Q] += -mul[q][C_C1] * fv[q][FT_A] * c9 * fv[q][FT_AH] * opw * RQ;
-          fb[q][FT_CA2] += mul[q][C_C1] * fv[q][FT_A] * c9 * fv[q][FT_AH] * opw * RQ;
-          fb[q][FT_AH] += -mul[q][C_C1] * fv[q][FT_A] * c9 * (cs2Q - ca2) * opw * RQ;
-          fb[q][FT_W] += -mul[q][C_C1] * fv[q][FT_A] * c9 * (cs2Q - ca2) * fv[q][FT_AH] * RQ;
-          fb[q][FT_RQ] += -mul[q][C_C1] * fv[q][FT_A] * c9 * (cs2Q - ca2) * fv[q][FT_AH] * opw;
-        }
-      }
-      break;
-    default:
-      break;
+    }
   }
 }
 
@@ -651,7 +594,21 @@ DEB_OUTLINE void feature_partials(const Ctx &cx, const uint32_t (&rd)[NS], const
       vq[q] = rsqrt_t<T>(1.0 + xq[q] * xq[q]);
       vb[q] = tconst<T>(0.0);
     }
-    row_feature_adj<K, T>(rd[s], cx, fv, vq, ms, acc, vb);
+    chain_feature_adj<K, T>(rd[s], cx, fv, vq, ms, acc, vb);
+    if (s == 0) {
+      T sp[K][FT_N];
+#pragma unroll
+      for (int q = 0; q < K; ++q)
+#pragma unroll
+        for (int m = 0; m < FT_N; ++m) sp[q][m] = tconst<T>(0.0);
+      special_feature_adj<K, T>(cx, fv, ms, sp);
+      if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < K; ++q)
+#pragma unroll
+          for (int m = 0; m < FT_N; ++m) acc[q][m] += sp[q][m];
+      }
+    }
     if (qrow) {
 #pragma unroll
       for (int q = 0; q < K; ++q) {

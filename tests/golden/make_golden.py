@@ -8,9 +8,13 @@ and the attempted / accepted step counts, at rtol 1e-5 (atol 1e-8) and -- for
 k <= 0.1 /Mpc, where the oracle finishes in seconds -- at rtol 1e-10
 (atol 1e-14), and P(k) at rtol 1e-6 (the result's sensitivity to the
 tolerance).  The gradient configuration adds dP/dtheta by central
-differences (relative step 1e-4) of rtol 1e-10 solves.
+differences (relative step 1e-4) of rtol 1e-10 solves, and (``--tangent``)
+the oracle's forward-mode tangent of the same rtol 1e-10 solve along the
+input tangents of the six directions (the derivative the reverse sweep
+computes; the differences also carry the table producer's own truncation).
 
     python tests/golden/make_golden.py [config ...]
+    python tests/golden/make_golden.py --tangent c4-grad-base
 """
 from __future__ import annotations
 
@@ -26,7 +30,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path[:0] = [os.path.join(ROOT, "disco-eb_amd"), os.path.join(ROOT, "oracle")]
 
 import oracle  # noqa: E402
-from discoeb_amd import packing  # noqa: E402
+from discoeb_amd import packing, tangents  # noqa: E402
 from discoeb_amd.background import evolve_background  # noqa: E402
 from discoeb_amd.cosmology import base_theta, loss_weights, param_from_theta, sample_thetas, SEED  # noqa: E402
 
@@ -131,6 +135,42 @@ def fd_gradient(theta, c, k):
     return g
 
 
+def _tangent_dir(args):
+    name, j = args
+    path = os.path.join(HERE, f"golden_{name}.json")
+    with open(path) as f:
+        g = json.load(f)
+    th = np.asarray(g["thetas"])
+    c = g["cosmologies"][0]
+    k = np.asarray(g["k"])[np.asarray(c["tight_kidx"])]
+    param, dcos, dtab, dtau = tangents.input_tangents(th, g["aexp"])
+    cfg = packing.make_config(rtol=TIGHT[0], atol=TIGHT[1], max_steps=2_000_000, **g["layout"])
+    prob = oracle.Problem(param, cfg)
+    tau = packing.tau_of_aexp(param, g["aexp"])[0]
+    r = prob.evolve_tangent(k, tau, dtau[0, j], dcos[0, j], dtab[0, j])
+    dm = r["y"][:, -1, 4]
+    # P = pref dm^2 with pref independent of the six directions
+    return (2.0 * power(th[0], k, dm) / dm * r["dy"][:, -1, 4]).tolist()
+
+
+def add_tangent(name):
+    """dP/dtheta of the rtol 1e-10 solve by the oracle's forward-mode tangent
+    (one process per direction)."""
+    from multiprocessing import Pool
+    with Pool(len(FD_DIRS)) as pool:
+        cols = pool.map(_tangent_dir, [(name, j) for j in range(len(FD_DIRS))])
+    path = os.path.join(HERE, f"golden_{name}.json")
+    with open(path) as f:
+        g = json.load(f)
+    g["cosmologies"][0]["dP_dtheta_tangent"] = np.asarray(cols).T.tolist()
+    with open(path, "w") as f:
+        json.dump(g, f, separators=(",", ":"))
+
+
 if __name__ == "__main__":
-    for name in (sys.argv[1:] or list(CONFIGS)):
-        make(name)
+    if sys.argv[1:2] == ["--tangent"]:
+        for name in sys.argv[2:]:
+            add_tangent(name)
+    else:
+        for name in (sys.argv[1:] or list(CONFIGS)):
+            make(name)
