@@ -861,15 +861,28 @@ __device__ __forceinline__ void rodas5_step_vjp(Mode<NS> &m, Ctx &cx, double t0,
       u[s] = y0[s];
       ck[s] = 0.0;
     }
-    for (int j = 0; j < i; ++j) {
-      const double aij = kA[i][j], cij = kC[i][j];
+    // the earlier stage vectors in one batch of loads (the tableau's rows are
+    // zero from column i on, so the sums over all seven add exact zeros)
+    {
+      double kj[7][NS];
 #pragma unroll
-      for (int s = 0; s < NS; ++s) {
-        const double kj = Kp(j, s);
-        u[s] += aij * kj;
-        ck[s] += cij * kj;
-        KBp(j, s) += (cij * ih) * rb[s];
+      for (int j = 0; j < 7; ++j)
+#pragma unroll
+        for (int s = 0; s < NS; ++s) kj[j][s] = Kp(j, s);
+#pragma unroll
+      for (int j = 0; j < 7; ++j) {
+        const double aij = kA[i][j], cij = kC[i][j];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+          u[s] += aij * kj[j][s];
+          ck[s] += cij * kj[j][s];
+        }
       }
+    }
+    for (int j = 0; j < i; ++j) {
+      const double cij = kC[i][j];
+#pragma unroll
+      for (int s = 0; s < NS; ++s) KBp(j, s) += (cij * ih) * rb[s];
     }
     const double ti = t0 + kc[i] * h;
     const double iti = 1.0 / ti;

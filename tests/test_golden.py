@@ -47,6 +47,23 @@ def test_oracle_reproduces_golden(name):
         np.testing.assert_allclose(P, c["P"], rtol=1e-14)
 
 
+def test_gradient_fixture_tangent_agrees_with_differences():
+    """The c4 fixture's two derivatives of the rtol 1e-10 P(k): the oracle's
+    forward-mode tangent and central differences of the whole pipeline (tables
+    included, relative step 1e-4).  They agree to 1e-5 below k = 0.01 /Mpc and
+    to 5e-3 at k = 0.1 /Mpc (omega_b there 3.0e-3: the differences' own
+    truncation through the recombination tables)."""
+    g = _load("c4-grad-base")
+    c = g["cosmologies"][0]
+    k = np.asarray(g["k"])[np.asarray(c["tight_kidx"])]
+    tan = np.asarray(c["dP_dtheta_tangent"])
+    fd = np.asarray(c["dP_dtheta_fd"])
+    assert tan.shape == fd.shape == (k.size, len(c["fd_dirs"]))
+    rel = np.abs(tan / fd - 1)
+    assert rel[k < 0.01].max() < 1e-5
+    assert rel.max() < 5e-3
+
+
 def test_flop_model_scales_with_the_layout():
     """The roofline numerator grows linearly with the state size at a fixed
     number of cyclic-reduction levels, the reverse step costs several forward
