@@ -788,14 +788,14 @@ __device__ __forceinline__ TD seed_td(double v, double d0) {
 // which must stay out of non-kernel functions)
 template <int NS>
 __device__ __forceinline__ void step_store(Mode<NS> &m, Ctx &cx, double t0, double h, const double (&y0)[NS],
-                                           double (&y1)[NS], double *K) {
+                                           double (&y1)[NS], double *K, lds_double *kl) {
   const int lane = lane_id();
   double err[NS];
   auto store = [&](int i, const double (&k)[NS]) {
 #pragma unroll
     for (int s = 0; s < NS; ++s) K[(i * NS + s) * WAVE + lane] = k[s];
   };
-  rodas5_step_s<NS, decltype(store), true>(m, cx, t0, h, y0, y1, err, store);
+  rodas5_step_s<NS, decltype(store), true>(m, cx, t0, h, y0, y1, err, store, kl);
 }
 
 // ---------------------------------------------------------------------------
@@ -849,10 +849,7 @@ __device__ __forceinline__ void rodas5_step_vjp(Mode<NS> &m, Ctx &cx, double t0,
   for (int i = 7; i >= 0; --i) {
     double rb[NS], ki[NS];
 #pragma unroll
-    for (int s = 0; s < NS; ++s) {
-      rb[s] = (i == 7) ? y1b[s] * kA8[7] : KBp(i, s);
-      ki[s] = Kp(i, s);
-    }
+    for (int s = 0; s < NS; ++s) rb[s] = (i == 7) ? y1b[s] * kA8[7] : KBp(i, s);
     wsolveT<NS>(m, rb);
     // stage point u_i and the combination sum_j C_ij k_j
     double u[NS], ck[NS];
@@ -861,14 +858,17 @@ __device__ __forceinline__ void rodas5_step_vjp(Mode<NS> &m, Ctx &cx, double t0,
       u[s] = y0[s];
       ck[s] = 0.0;
     }
-    // the earlier stage vectors in one batch of loads (the tableau's rows are
-    // zero from column i on, so the sums over all seven add exact zeros)
+    // this stage's vector and the earlier ones in one batch of loads (the
+    // tableau's rows are zero from column i on, so the sums over all seven add
+    // exact zeros)
     {
       double kj[7][NS];
 #pragma unroll
       for (int j = 0; j < 7; ++j)
 #pragma unroll
         for (int s = 0; s < NS; ++s) kj[j][s] = Kp(j, s);
+#pragma unroll
+      for (int s = 0; s < NS; ++s) ki[s] = Kp(i, s);
 #pragma unroll
       for (int j = 0; j < 7; ++j) {
         const double aij = kA[i][j], cij = kC[i][j];
@@ -1293,7 +1293,7 @@ __device__ __forceinline__ void adjoint_mode(const Params<NS> &P, const AdjArgs 
         double *ry = ring_y(j);
 #pragma unroll
         for (int s = 0; s < NS; ++s) ry[s * WAVE + lane] = y[s];
-        step_store<NS>(m, cx, tt[j], tt[j + 1] - tt[j], y, y1, ring_k(j));
+        step_store<NS>(m, cx, tt[j], tt[j + 1] - tt[j], y, y1, ring_k(j), (lds_double *)KBl);
 #pragma unroll
         for (int s = 0; s < NS; ++s) y[s] = y1[s];
       }

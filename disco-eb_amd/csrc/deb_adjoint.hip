@@ -169,7 +169,7 @@ __global__ void __launch_bounds__(WAVE) deb_step_vjp_kernel(const Params<NS> P, 
   for (int j = 0; j < MAXDIR; ++j) thb[j] = 0.0;
   double tb = 0.0, hb = 0.0;
   double y1[NS];
-  step_store<NS>(m, cx, t0, h, y, y1, Kl);
+  step_store<NS>(m, cx, t0, h, y, y1, Kl, (lds_double *)KBl);
   rodas5_step_vjp<NS>(m, cx, t0, h, y, yb, thb, tb, hb, Kl, KBl);
   double *yo = H.ybar + (long long)item * P.n;
 #pragma unroll

@@ -292,7 +292,10 @@ template <int NS, bool TAPE = false>
 __device__ __forceinline__ void integrate_mode(const Params<NS> &P, int ci, int ki, double *lds, lds_double *pcr,
                                                const TapeView *tape = nullptr) {
   const int lane = lane_id();
-  Ctx cx;
+  // the mode's context record in the wave's LDS rather than in the register
+  // file, which the inlined step fills to its limit (spilling to scratch)
+  __shared__ Ctx cxs;
+  Ctx &cx = cxs;
   const double k = P.kmodes[(int64_t)ci * P.nk + ki];
   make_ctx<NS>(cx, P, ci, k);
   Mode<NS> m;
@@ -409,7 +412,8 @@ __device__ __forceinline__ void integrate_batch(const Params<NS> &P, int ci, int
                                                 lds_double *pcr, double *ws) {
   const int lane = lane_id();
   const int64_t mode0 = (int64_t)ci * P.nk + kb;
-  Ctx cx;
+  __shared__ Ctx cxs;  // in LDS, as in integrate_mode
+  Ctx &cx = cxs;
   make_ctx<NS>(cx, P, ci, P.kmodes[mode0]);
   Mode<NS> m;
   m.pcr = pcr;
