@@ -302,10 +302,12 @@ struct RowSumsT {
 };
 using RowSums = RowSumsT<double>;
 
-template <int NS, bool TAPPLY>
+// With DFA, also the features' derivatives along a (dfa_out), from a dual
+// evaluation beside the plain one (the plain values feed the rows unchanged).
+template <int NS, bool TAPPLY, bool DFA = false>
 DEB_OUTLINE RowSums coef_pass(Ctx &cx, const uint32_t (&rd)[NS], double a, double invtau,
                               const double (&w_in)[NS], const double (&Y_in)[NS], double (&out)[NS],
-                              double (&fv_out)[FT_N]) {
+                              double (&fv_out)[FT_N], double *dfa_out = nullptr) {
   const int lane = lane_id();
   double w[NS], Y[NS];
 #pragma unroll
@@ -318,6 +320,15 @@ DEB_OUTLINE RowSums coef_pass(Ctx &cx, const uint32_t (&rd)[NS], double a, doubl
   features_of<double>(cx, a, b, fv);
 #pragma unroll
   for (int m = 0; m < FT_N; ++m) fv_out[m] = fv[m];
+  if constexpr (DFA) {
+    dual ad;
+    ad.v = a;
+    ad.d = 1.0;
+    dual Fd[FT_N];
+    features_t<dual>(cx, ad, Fd);
+#pragma unroll
+    for (int m = 0; m < FT_N; ++m) dfa_out[m] = Fd[m].d;
+  }
   double o[NS];
   double part[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   double cc[4][NS];
@@ -680,6 +691,23 @@ DEB_OUTLINE double lane_contract(const Ctx &cx, double a, const double (&fbs)[FT
   return lane_dot(F, fbs);
 }
 
+// The input-tangent directions of sum_m F_m(a_i) fbs_im for the eight stages
+// of a reverse step at once: lane l evaluates the features at stage l >> 3's
+// a along direction (l & 7) - 1 (none on l & 7 == 0).  cst: the stages' a_i
+// (8 doubles), then their wave-uniform partials fbs_i (8 x FT_N).
+DEB_OUTLINE double lane_contract_stages(const Ctx &cx, const lds_double *cst) {
+  Ctx cl = lane_ctx(cx);
+  const int i = lane_id() >> 3;
+  TL as;
+  as = (double)cst[i];
+  TL F[FT_N];
+  features_t<TL>(cl, as, F);
+  double g = 0.0;
+#pragma unroll
+  for (int m = 0; m < FT_N; ++m) g += F[m].d[1] * (double)cst[8 + i * FT_N + m];
+  return g;
+}
+
 template <class T>
 DEB_OUTLINE void features_call(Ctx &cx, T a, T (&F)[FT_N]) {
   T f[FT_N];
@@ -811,6 +839,10 @@ __device__ __forceinline__ void rodas5_step_vjp(Mode<NS> &m, Ctx &cx, double t0,
   const int lane = lane_id();
   auto Kp = [&](int i, int s) -> double { return K[(i * NS + s) * WAVE + lane]; };
   auto KBp = [&](int i, int s) -> double & { return KBl[(i * NS + s) * WAVE + lane]; };
+  // per stage: its a and its wave-uniform feature partials (written by every
+  // lane alike)
+  __shared__ double cst_[8 * (1 + FT_N)];
+  lds_double *const cst = (lds_double *)cst_;
   factor_call<NS>(m, cx, t0, y0, h);
   double tw[NS];
   trunc_weights<NS>(m.rd, tw);
@@ -909,7 +941,8 @@ __device__ __forceinline__ void rodas5_step_vjp(Mode<NS> &m, Ctx &cx, double t0,
       const double ai = readlane(u[0], L_A);
       double Yq[2][NS], wq[2][NS];
       RowSums rsq[2];
-      rsq[0] = coef_pass<NS, true>(cx, m.rd, ai, iti, rb, u, ub, fvs[0]);
+      double dfa[FT_N];
+      rsq[0] = coef_pass<NS, true, true>(cx, m.rd, ai, iti, rb, u, ub, fvs[0], dfa);
 #pragma unroll
       for (int s = 0; s < NS; ++s) {
         Yq[0][s] = u[s];
@@ -925,13 +958,15 @@ __device__ __forceinline__ void rodas5_step_vjp(Mode<NS> &m, Ctx &cx, double t0,
 #pragma unroll
       for (int q = 0; q < FT_N; ++q) fb[q] = fbs[0][q];
       wave_sum<FT_N>(fb);
-      const double g = lane_contract(cx, ai, fb);
-      const double ga = readlane(g, 0);
-      if (lane == L_A) ub[0] += ga;
-      double tl[MAXDIR];
-      lane_gather(g, cx.ndir, tl);
+      // the a-direction now (it feeds the earlier stages); the input-tangent
+      // directions of all stages in one pass after the loop
+      double ga = 0.0;
 #pragma unroll
-      for (int j = 0; j < MAXDIR; ++j) th_acc[j] += tl[j];
+      for (int q = 0; q < FT_N; ++q) ga += dfa[q] * fb[q];
+      if (lane == L_A) ub[0] += ga;
+      cst[i] = ai;
+#pragma unroll
+      for (int q = 0; q < FT_N; ++q) cst[8 + i * FT_N + q] = fb[q];
       const double kia = readlane(ki[0], L_A);
 #pragma unroll
       for (int s = 0; s < NS; ++s) rho[s] += kia * rb[s];
@@ -942,6 +977,19 @@ __device__ __forceinline__ void rodas5_step_vjp(Mode<NS> &m, Ctx &cx, double t0,
       const double aij = kA[i][j];
 #pragma unroll
       for (int s = 0; s < NS; ++s) KBp(j, s) += aij * ub[s];
+    }
+  }
+  // the stages' input-tangent directions
+  {
+    const double g = lane_contract_stages(cx, cst);
+#pragma unroll
+    for (int j = 0; j < MAXDIR; ++j) {
+      if (j < cx.ndir) {
+        double t = 0.0;
+#pragma unroll
+        for (int st = 0; st < 8; ++st) t += readlane(g, st * 8 + j + 1);
+        th_acc[j] += t;
+      }
     }
   }
   // the J(t0, y0) k_i terms, contracted with the feature tangents at a0
