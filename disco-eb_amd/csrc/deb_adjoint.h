@@ -220,7 +220,7 @@ DEB_OUTLINE void tapply(Ctx &cx, const uint32_t (&rd)[NS], T a, double invtau, c
   T cc[4][NS];
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
-    Coef<T> r = row_coef<T>(rd[s], b, cx, invtau);
+    Coef<T> r = row_coef_slot<T>(rd[s], b, cx, invtau, s == 0);
     T sw = r.sub * w[s];
     T pw = r.sup * w[s];
     out[s] = r.diag * w[s] + t_from_next(sw) + t_from_prev(pw);
@@ -334,7 +334,7 @@ DEB_OUTLINE RowSums coef_pass(Ctx &cx, const uint32_t (&rd)[NS], double a, doubl
   double cc[4][NS];
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
-    Coef<double> r = row_coef<double>(rd[s], b, cx, invtau);
+    Coef<double> r = row_coef_slot<double>(rd[s], b, cx, invtau, s == 0);
     part[0] += r.ch * Y[s];
     part[1] += r.ceta * Y[s];
     part[2] += r.cpi * Y[s];
@@ -733,7 +733,7 @@ DEB_OUTLINE void row_sums_t(Ctx &cx, const uint32_t (&rd)[NS], T a, double invta
   for (int i = 0; i < 7; ++i) part[i] = tconst<T>(0.0);
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
-    const Coef<T> r = row_coef<T>(rd[s], b, cx, invtau);
+    const Coef<T> r = row_coef_slot<T>(rd[s], b, cx, invtau, s == 0);
     part[0] = part[0] + r.ch * Y[s];
     part[1] = part[1] + r.ceta * Y[s];
     part[2] = part[2] + r.cpi * Y[s];

@@ -292,10 +292,7 @@ template <int NS, bool TAPE = false>
 __device__ __forceinline__ void integrate_mode(const Params<NS> &P, int ci, int ki, double *lds, lds_double *pcr,
                                                const TapeView *tape = nullptr) {
   const int lane = lane_id();
-  // the mode's context record in the wave's LDS rather than in the register
-  // file, which the inlined step fills to its limit (spilling to scratch)
-  __shared__ Ctx cxs;
-  Ctx &cx = cxs;
+  Ctx cx;
   const double k = P.kmodes[(int64_t)ci * P.nk + ki];
   make_ctx<NS>(cx, P, ci, k);
   Mode<NS> m;
@@ -412,8 +409,7 @@ __device__ __forceinline__ void integrate_batch(const Params<NS> &P, int ci, int
                                                 lds_double *pcr, double *ws) {
   const int lane = lane_id();
   const int64_t mode0 = (int64_t)ci * P.nk + kb;
-  __shared__ Ctx cxs;  // in LDS, as in integrate_mode
-  Ctx &cx = cxs;
+  Ctx cx;
   make_ctx<NS>(cx, P, ci, P.kmodes[mode0]);
   Mode<NS> m;
   m.pcr = pcr;

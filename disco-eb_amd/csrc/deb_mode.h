@@ -399,4 +399,159 @@ __device__ __forceinline__ Coef<T> row_coef(uint32_t d, const Bg<T> &b, const Ct
   return r;
 }
 
+// row_coef organised for the wave instead of as a divergent switch, with the
+// same formulas in the same operation order (the host emulation compares the
+// two): the chain rows (hierarchies, any slot) by per-lane selects among their
+// formulas; with slot0, the fixed-lane rows of slot 0 as wave-uniform values
+// selected by lane.
+template <typename T>
+__device__ __forceinline__ Coef<T> row_coef_wave(uint32_t d, const Bg<T> &b, const Ctx &cx, double invtau,
+                                                 bool slot0) {
+  using P = ParT<T>;
+  const T zero = tconst<T>(0.0);
+  auto c = [](double v) { return tconst<T>(v); };
+  const P grhom = cv<T>(cx, F_GRHOM), grhog = cv<T>(cx, F_GRHOG), grhor = cv<T>(cx, F_GRHOR);
+  const P Neff = cv<T>(cx, F_NEFF), Nmnu = cv<T>(cx, F_NMNU);
+  const double k = cx.k, k2 = k * k;
+  const double ik2 = 1.0 / k2, hk2 = 0.5 * ik2;
+  const uint32_t t = rd_type(d);
+  const double l = (double)rd_l(d);
+  const bool F2 = t == R_F2, FL = t == R_FL, FLM = t == R_FLMAX, G0 = t == R_G0, GL = t == R_GL, G2 = t == R_G2,
+             GLM = t == R_GLMAX, DR = t == R_DR, TR = t == R_TR, FR2 = t == R_FR2, FRL = t == R_FRL,
+             FRLM = t == R_FRLMAX, Q0 = t == R_Q0, Q1 = t == R_Q1, Q2 = t == R_Q2, QL = t == R_QL,
+             QLM = t == R_QLMAX;
+  const bool qrow = t >= R_Q0 && t <= R_QLMAX;
+  const int qi = qrow ? (int)rd_q(d) : 0;
+  const T ia = b.ia, ia2 = b.ia2, iaH = b.iaH;
+  const double il = rcp_nr(2.0 * l + 1.0);
+  const double lsub = l * il, lsup = (l + 1.0) * il;
+  // v_q and 1/v_q of the row's bin (1 on other rows); k v_q multiplies the
+  // couplings of every chain (k v_q = k exactly where v_q = 1)
+  T vq = c(1.0), ivq = c(1.0);
+  {
+    const T aq = b.a * (cv<T>(cx, F_AMNU) * rcp_nr(cx.q[qi]));
+    const T s2 = 1.0 + aq * aq;
+    const T v = rsqrt_t<T>(s2);
+    vq = qrow ? v : c(1.0);
+    ivq = qrow ? s2 * v : c(1.0);
+  }
+  const T kv = k * vq;
+  const double wq = cx.w[qi], dl = cx.dlnf[qi];
+  Coef<T> r;
+  // sub-diagonal
+  double cs = 0.0;
+  cs = (FL || GL || FRL || QL) ? lsub : cs;
+  cs = (FLM || GLM || FRLM || QLM) ? 1.0 : cs;
+  cs = (G2 || Q2) ? 0.4 : cs;
+  cs = Q1 ? 1.0 / 3.0 : cs;
+  r.sub = kv * cs;
+  r.sub = (F2 || FR2) ? c(8.0 / 15.0) : r.sub;
+  r.sub = TR ? c(0.25 * k2) : r.sub;
+  // diagonal: opacity of the photon rows, free streaming out of the last multipole
+  const bool photon = F2 || FL || FLM || G0 || GL || G2 || GLM;
+  r.diag = photon ? -b.opac : zero;
+  r.diag = (FLM || GLM) ? -(l + 1.0) * invtau - b.opac : r.diag;
+  r.diag = (FRLM || QLM) ? c(-(l + 1.0) * invtau) : r.diag;
+  // super-diagonal
+  double cp = 0.0;
+  cp = (F2 || G2 || FR2 || Q2) ? 0.6 : cp;
+  cp = (FL || GL || FRL || QL) ? lsup : cp;
+  cp = (G0 || Q0) ? 1.0 : cp;
+  cp = Q1 ? 2.0 / 3.0 : cp;
+  r.sup = cp != 0.0 ? -(kv * cp) : zero;
+  r.sup = DR ? c(-4.0 / 3.0) : r.sup;
+  r.sup = TR ? c(-0.5 * k2) : r.sup;
+  // where h', eta', Pi enter
+  r.uh = (F2 || FR2) ? c(4.0 / 15.0) : zero;
+  r.uh = DR ? c(-2.0 / 3.0) : r.uh;
+  r.uh = Q0 ? c(dl * (1.0 / 6.0)) : r.uh;
+  r.uh = Q2 ? c(dl * (-1.0 / 15.0)) : r.uh;
+  r.ueta = (F2 || FR2) ? c(8.0 / 5.0) : (Q2 ? c(-0.4 * dl) : zero);
+  r.upi = (F2 || G2) ? 0.1 * b.opac : (G0 ? 0.5 * b.opac : zero);
+  // contributions to h', eta', Pi and f_1
+  const P rr = grhor * Neff;
+  const P rn = grhor * Nmnu * wq;
+  r.ch = DR ? rr * ia2 * iaH : (Q0 ? rn * ivq * ia2 * iaH : zero);
+  r.ceta = TR ? (4.0 / 3.0) * Neff * grhor * ia2 * hk2 : (Q1 ? Nmnu * grhor * k * wq * ia2 * hk2 : zero);
+  r.cpi = (F2 || G0 || G2) ? c(1.0) : zero;
+  r.c1 = DR ? (-2.0 * rr) * ia : (Q0 ? -(rn * (ivq + vq)) * ia : zero);
+  r.leaf = zero;
+  if (slot0) {
+    const int lane = lane_id();
+    const P Omegam = cv<T>(cx, F_OMEGAM), Omegab = cv<T>(cx, F_OMEGAB);
+    const P Omegac = Omegam - Omegab;
+    const T a = b.a;
+    // eta
+    r.ueta = lane == L_ETA ? c(1.0) : r.ueta;
+    r.ch = lane == L_ETA ? (2.0 * k2) * iaH : r.ch;
+    // delta_c, delta_b
+    const P rc = grhom * Omegac, rb = grhom * Omegab;
+    const bool dcb = lane == L_DC || lane == L_DB;
+    const P rcb = lane == L_DC ? rc : rb;
+    r.sup = dcb ? c(-1.0) : r.sup;
+    r.uh = dcb ? c(-0.5) : r.uh;
+    r.ch = dcb ? rcb * ia * iaH : r.ch;
+    r.c1 = dcb ? zero - rcb : r.c1;
+    // theta_c
+    r.diag = lane == L_TC ? -b.aH : r.diag;
+    r.ceta = lane == L_TC ? grhom * Omegac * ia * hk2 : r.ceta;
+    // theta_b
+    {
+      const T po = b.pb43 * b.opac;
+      const bool on = lane == L_TB;
+      r.sub = on ? k2 * b.cs2 : r.sub;
+      r.diag = on ? -b.aH - po : r.diag;
+      r.sup = on ? po : r.sup;
+      r.ceta = on ? grhom * Omegab * ia * hk2 : r.ceta;
+    }
+    // delta_gamma
+    {
+      const bool on = lane == L_DG;
+      r.leaf = on ? c(-4.0 / 3.0) : r.leaf;
+      r.uh = on ? c(-2.0 / 3.0) : r.uh;
+      r.ch = on ? grhog * ia2 * iaH : r.ch;
+      r.c1 = on ? (-2.0 * grhog) * ia : r.c1;
+    }
+    // theta_gamma
+    {
+      const bool on = lane == L_TG;
+      r.sub = on ? b.opac : r.sub;
+      r.diag = on ? -b.opac : r.diag;
+      r.sup = on ? c(-0.5 * k2) : r.sup;
+      r.leaf = on ? c(0.25 * k2) : r.leaf;
+      r.ceta = on ? (4.0 / 3.0) * grhog * ia2 * hk2 : r.ceta;
+    }
+    // dark-energy fluid (frozen for a cosmological constant)
+    if (!cx.lambda) {
+      const P OmegaDE = cv<T>(cx, F_OMEGADE), cs2Q = cv<T>(cx, F_CS2DE);
+      const T opw = 1.0 + b.w;
+      const T rq = grhom * OmegaDE * b.rhoQ * (a * a);
+      const bool dq = lane == L_DQ, tq = lane == L_TQ;
+      r.diag = dq ? -3.0 * (cs2Q - b.w) * b.aH : r.diag;
+      r.sup = dq ? -opw - 9.0 * opw * (cs2Q - b.ca2) * b.aH * b.aH * ik2 : r.sup;
+      r.uh = dq ? -0.5 * opw : r.uh;
+      r.ch = dq ? rq * iaH : r.ch;
+      r.c1 = dq ? -a * (rq + 3.0 * cs2Q * rq) : r.c1;
+      r.sub = tq ? cs2Q / opw * k2 : r.sub;
+      r.diag = tq ? -(1.0 - 3.0 * cs2Q) * b.aH : r.diag;
+      r.ceta = tq ? opw * rq * hk2 : r.ceta;
+      r.c1 = tq ? -a * (3.0 * (cs2Q - b.ca2) * (3.0 * b.aH * ik2) * opw * rq) : r.c1;
+    }
+  }
+  return r;
+}
+
+// row_coef of slot s: the wave form in plain doubles; dual arithmetic (the
+// factorisation's a-column and the reverse step's a-column) keeps the switch.
+// With the wave form also in the factorisation's dual right-hand side, the
+// fully inlined forward kernel ended 41 % of the headline modes at max_steps on
+// the GPU, although the host emulation finds the dual wave form equal to the
+// switch in every field; the cause was not found.
+template <typename T>
+__device__ __forceinline__ Coef<T> row_coef_slot(uint32_t d, const Bg<T> &b, const Ctx &cx, double invtau,
+                                                 bool slot0) {
+  if constexpr (std::is_same<T, double>::value) return row_coef_wave<T>(d, b, cx, invtau, slot0);
+  else return row_coef<T>(d, b, cx, invtau);
+}
+
 }  // namespace deb

@@ -12,7 +12,8 @@
 //   simt_host <in.bin> <out.bin>
 //
 // in.bin (little-endian, written by tests/test_simt_host.py):
-//   int32 mode (0: forward solve, 1: taped forward + reverse sweep)
+//   int32 mode (0: forward solve, 1: taped forward + reverse sweep, 2: the
+//   row coefficients' wave form against the row switch)
 //   deb_config_t cfg; deb_cosmo_t cosmo; int64 stride; double tables[stride];
 //   int32 desc[14]; int32 nq; double q[max(nq,1)], w[max(nq,1)];
 //   int32 nk; double k[nk]; int32 nout; double tau[nout];
@@ -20,9 +21,11 @@
 //                double dtab[ndir][stride]; double dtau[ndir][nout];
 //                double ybar[nk][nout][20]
 // out.bin: int32 status[nk]; int32 nsteps[nk]; double y[nk][nout][width];
-//   mode 1 adds: double grad[nk][ndir]
+//   mode 1 adds: double grad[nk][ndir]; mode 2 writes only int32 mismatches,
+//   int32 rows checked
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <string>
 #include <thread>
 #include <vector>
@@ -86,6 +89,47 @@ int run(const deb_config_t &cfg, const Layout &L, Reader &in, FILE *out, int mod
   P.y_out = y.data();
   P.status = status.data();
   P.nsteps = nsteps.data();
+  if (mode == 2) {
+    // the wave form of the row coefficients against the row switch, bit for
+    // bit, for every lane and slot at several a and both dark-energy modes
+    int32_t bad = 0, checked = 0;
+    for (int ki = 0; ki < nk; ++ki) {
+      run_wave([&](int lane) {
+        Ctx cx;
+        make_ctx<NS>(cx, P, 0, k[ki]);
+        uint32_t rd[NS];
+        for (int s = 0; s < NS; ++s) rd[s] = P.rowdesc[s][lane];
+        int nb = 0, nc = 0;
+        for (int lam = 0; lam < 2; ++lam) {
+          cx.lambda = lam == 1;
+          for (double a = 1e-7; a <= 1.0; a *= 7.0) {
+            const Bg<double> b = background<double>(cx, a);
+            const double invtau = 1.0 / (30.0 + 1e4 * a);
+            dual ad;
+            ad.v = a;
+            ad.d = 1.0;
+            const Bg<dual> bd = background<dual>(cx, ad);
+            for (int s = 0; s < NS; ++s) {
+              const Coef<double> r0 = row_coef<double>(rd[s], b, cx, invtau);
+              const Coef<double> r1 = row_coef_wave<double>(rd[s], b, cx, invtau, s == 0);
+              nb += std::memcmp(&r0, &r1, sizeof(r0)) != 0;
+              // in dual arithmetic along a: equal values and derivatives
+              const Coef<dual> d0 = row_coef<dual>(rd[s], bd, cx, invtau);
+              const Coef<dual> d1 = row_coef_wave<dual>(rd[s], bd, cx, invtau, s == 0);
+              const dual *p0 = &d0.sub, *p1 = &d1.sub;
+              for (int f = 0; f < 11; ++f) nb += !(p0[f].v == p1[f].v && p0[f].d == p1[f].d);
+              nc += 2;
+            }
+          }
+        }
+        __atomic_add_fetch(&bad, nb, __ATOMIC_RELAXED);
+        __atomic_add_fetch(&checked, nc, __ATOMIC_RELAXED);
+      });
+    }
+    fwrite(&bad, 4, 1, out);
+    fwrite(&checked, 4, 1, out);
+    return 0;
+  }
   if (mode == 0) {
     for (int ki = 0; ki < nk; ++ki) {
       std::vector<double> lds(fwd_lds_doubles<NS>(), -777.0), pcr(pcr_lds_doubles<NS>(), -444.0);
@@ -183,6 +227,7 @@ int main(int argc, char **argv) {
     case 1: r = run<1>(cfg, L, in, out, mode, cosmo, tables, stride, desc, q, w, k, tau); break;
     case 2: r = run<2>(cfg, L, in, out, mode, cosmo, tables, stride, desc, q, w, k, tau); break;
     case 4: r = run<4>(cfg, L, in, out, mode, cosmo, tables, stride, desc, q, w, k, tau); break;
+    case 5: r = run<5>(cfg, L, in, out, mode, cosmo, tables, stride, desc, q, w, k, tau); break;
     default: fprintf(stderr, "simt_host: slot count %d not built\n", L.ns); return 2;
   }
   fclose(out);

@@ -114,3 +114,26 @@ def test_device_code_under_sanitizers_matches_oracle(harness, test_param, grad_i
         want = r["dy"][0, :, 4].sum()
         scale = np.abs(r["dy"][0, :, 4]).sum() + 1e-30
         assert abs(grad[0, j] - want) <= 3e-3 * scale, (j, grad[0, j], want)
+
+
+def test_row_coefficient_wave_form_is_bit_identical(harness, test_param, tmp_path):
+    """The row coefficients' wave form (row_coef_wave: per-lane selects and the
+    fixed-lane rows of slot 0 as wave-uniform values) equals the row switch
+    (row_coef) bit for bit on every lane and slot, at seven a in [1e-7, 1],
+    with and without the dark-energy fluid, for three layouts (lmax 11 with
+    3 bins, lmax 32 with 5 bins, lmax 50)."""
+    layouts = {"ns2": dict(lmaxg=11, lmaxgp=11, lmaxr=11, lmaxnu=8, nqmax=3),
+               "ns4": dict(lmaxg=32, lmaxgp=32, lmaxr=32, lmaxnu=17, nqmax=5),
+               "ns5": dict(lmaxg=50, lmaxgp=50, lmaxr=50, lmaxnu=17, nqmax=5)}
+    tau = packing.tau_of_aexp(test_param, AOUT)[0]
+    kk = np.array([1e-4, 3e-2, 2.0])
+    procs = {}
+    for name, lay in layouts.items():
+        cfg = packing.make_config(rtol=1e-4, atol=1e-8, max_steps=2000, **lay)
+        prob = oracle.Problem(test_param, cfg)
+        _write(tmp_path / f"rc_{name}.in", 2, cfg, prob, kk, tau)
+        procs[name] = _start(harness, tmp_path, f"rc_{name}")
+    for name, p in procs.items():
+        _finish(p, 600)
+        bad, checked = np.frombuffer((tmp_path / f"rc_{name}.out").read_bytes(), np.int32)
+        assert checked > 0 and bad == 0, (name, int(bad), int(checked))
