@@ -53,7 +53,13 @@ __device__ __forceinline__ void eval_rhs(Ctx &cx, const uint32_t (&rd)[NS], doub
   const double invtau = 1.0 / tau;
   Coef<T> cf[NS];
 #pragma unroll
-  for (int s = 0; s < NS; ++s) cf[s] = row_coef_slot<T>(rd[s], b, cx, invtau, s == 0);
+  for (int s = 0; s < NS; ++s) {
+    // five slots (lmax 50) keep the switch: there the stage vectors stay in
+    // registers, and the selects' extra live values cost more than the
+    // branches they save
+    if constexpr (NS <= 4) cf[s] = row_coef_slot<T>(rd[s], b, cx, invtau, s == 0);
+    else cf[s] = row_coef<T>(rd[s], b, cx, invtau);
+  }
   // functionals h', eta', Pi, f_1
   constexpr int NV = (sizeof(T) == sizeof(double)) ? 4 : 8;
   double part[NV];
