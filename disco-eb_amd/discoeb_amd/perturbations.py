@@ -7,7 +7,7 @@ Same keyword-only signatures, defaults and return arity as the reference:
 * ``get_power``                     src/discoeb/perturbations.py:1101-1123
 
 plus ``evolve_perturbations_multi``: many cosmologies x k-modes in one
-persistent launch.  Results are torch FP64 tensors on the GPU.  There is no
+launch (one wavefront per mode, largest k first).  Results are torch FP64 tensors on the GPU.  There is no
 CPU fallback: a missing GPU or library raises.
 """
 from __future__ import annotations
@@ -170,7 +170,7 @@ def evolve_perturbations_multi(*, param, aexp_out, kmodes, lmaxg: int = 11, lmax
                                factormax: float = 20.0, factormin: float = 0.3, max_steps: int = 2048,
                                return_full: bool = False, throw: bool = True):
     """All (cosmology, k) modes of a batched ``param`` (B cosmologies) in one
-    persistent launch.  ``kmodes`` is (nk,) or (B, nk).  Returns
+    launch.  ``kmodes`` is (nk,) or (B, nk).  Returns
     ``(y[B, nk, nout, width], status[B, nk], nsteps[B, nk])``."""
     torch = _torch()
     cfg = _config(lmaxg, lmaxgp, lmaxr, lmaxnu, nqmax, rtol, atol, pcoeff, icoeff, dcoeff, factormax, factormin,
