@@ -343,9 +343,9 @@ struct R5 {
                           d4 = 0.3449828624725343, d5 = -0.377417564392089818;
 };
 
-// Out-of-line versions of the step's three parts, for kernels that run the
-// step next to much other live state (the reverse kernel): as calls, the step
-// factors stay in memory instead of registers.
+// Out-of-line versions of the factorisation and the right-hand side, for
+// kernels that run the step next to much other live state (the reverse
+// kernel): as calls, the step factors stay in memory instead of registers.
 template <int NS>
 __device__ __noinline__ void factor_call(Mode<NS> &m, Ctx &cx, double tau0, const double (&y0)[NS], double h) {
   factor<NS>(m, cx, tau0, y0, h);
@@ -367,19 +367,10 @@ __device__ __noinline__ void rhs_call(Ctx &cx, const uint32_t (&rd)[NS], double 
 #pragma unroll
   for (int s = 0; s < NS; ++s) f_out[s] = f[s];
 }
-template <int NS>
-__device__ __noinline__ void wsolve_call(const Mode<NS> &m, double (&r_io)[NS]) {
-  double r[NS];
-#pragma unroll
-  for (int s = 0; s < NS; ++s) r[s] = r_io[s];
-  wsolve<NS>(m, r);
-#pragma unroll
-  for (int s = 0; s < NS; ++s) r_io[s] = r[s];
-}
 
 // one Rodas5Transformed step (ode_integrators_stiff.py:718-834): y1 and err = k8;
 // st(i, k) sees every stage vector k_{i+1} once it is solved.  CALLS selects
-// the out-of-line factorisation, right-hand side and solve.  With kl (the
+// the out-of-line factorisation and right-hand side.  With kl (the
 // wave's LDS, 7 x NS x 64 doubles) the stage vectors k1..k7 are kept there
 // instead of in registers, which relieves the register file of the kernels
 // that run the step fully inlined.
@@ -393,10 +384,9 @@ __device__ __forceinline__ void rodas5_step_s(Mode<NS> &m, Ctx &cx, double t0, d
     if constexpr (CALLS) rhs_call<NS>(cx, m.rd, tau, y, f);
     else eval_rhs<NS, double>(cx, m.rd, tau, y, f);
   };
-  auto wsolve_ = [&](double (&r)[NS]) {
-    if constexpr (CALLS) wsolve_call<NS>(m, r);
-    else wsolve<NS>(m, r);
-  };
+  // the solves stay inline in both forms (as a call each, the recompute of the
+  // reverse kernel measured 2 % slower)
+  auto wsolve_ = [&](double (&r)[NS]) { wsolve<NS>(m, r); };
   if constexpr (CALLS) factor_call<NS>(m, cx, t0, y0, dt);
   else factor<NS>(m, cx, t0, y0, dt);
   double dT[NS];
