@@ -201,50 +201,6 @@ DEB_OUTLINE void wsolveT(const Mode<NS> &m, double (&xio)[NS]) {
 // ---------------------------------------------------------------------------
 // right-hand side contractions
 
-// out = A(a)^T w for the y-linear part of f (rows other than a; the a-column
-// and the a-row are handled through the features).  T = dual gives
-// (A^T w, A'(a)^T w).
-template <int NS, class T>
-DEB_OUTLINE void tapply(Ctx &cx, const uint32_t (&rd)[NS], T a, double invtau, const double (&w_in)[NS],
-                        T (&out_io)[NS]) {
-  const int lane = lane_id();
-  double w[NS];
-  T out[NS];
-#pragma unroll
-  for (int s = 0; s < NS; ++s) w[s] = w_in[s];
-  Bg<T> b = background<T>(cx, a);
-  T su[3];
-  su[0] = tconst<T>(0.0);
-  su[1] = tconst<T>(0.0);
-  su[2] = tconst<T>(0.0);
-  T cc[4][NS];
-#pragma unroll
-  for (int s = 0; s < NS; ++s) {
-    Coef<T> r = row_coef_slot<T>(rd[s], b, cx, invtau, s == 0);
-    T sw = r.sub * w[s];
-    T pw = r.sup * w[s];
-    out[s] = r.diag * w[s] + t_from_next(sw) + t_from_prev(pw);
-    if (s == 0) {
-      T lw = r.leaf * w[0];
-      T l_dg = t_readlane(lw, L_DG), l_tg = t_readlane(lw, L_TG);
-      if (lane == L_TG) out[0] = out[0] + l_dg;
-      if (lane == L_DG) out[0] = out[0] + l_tg;
-    }
-    su[0] = su[0] + r.uh * w[s];
-    su[1] = su[1] + r.ueta * w[s];
-    su[2] = su[2] + r.upi * w[s];
-    cc[0][s] = r.ch;
-    cc[1][s] = r.ceta;
-    cc[2][s] = r.cpi;
-    cc[3][s] = r.c1;
-  }
-  t_wave_sum<T, 3>(su);
-  const double wahp = readlane(w[0], L_AHP);
-#pragma unroll
-  for (int s = 0; s < NS; ++s)
-    out_io[s] = out[s] + cc[0][s] * su[0] + cc[1][s] * su[1] + cc[2][s] * su[2] + cc[3][s] * wahp;
-}
-
 // ---------------------------------------------------------------------------
 // First-order contraction through background features.
 //
@@ -708,19 +664,17 @@ DEB_OUTLINE double lane_contract_stages(const Ctx &cx, const lds_double *cst) {
   return g;
 }
 
-template <class T>
-DEB_OUTLINE void features_call(Ctx &cx, T a, T (&F)[FT_N]) {
-  T f[FT_N];
-  features_t<T>(cx, a, f);
-#pragma unroll
-  for (int m = 0; m < FT_N; ++m) F[m] = f[m];
-}
-
-// the functional sums of sum_rows w f_row(a, Y) in T (for T = dual along a:
-// their values and a-derivatives)
-template <int NS, class T>
-DEB_OUTLINE void row_sums_t(Ctx &cx, const uint32_t (&rd)[NS], T a, double invtau, const double (&w_in)[NS],
-                            const double (&Y_in)[NS], RowSumsT<T> &rs) {
+// The a-column pass of the reverse step in dual arithmetic along a, with one
+// background evaluation for its three parts: out = A(a)^T w for the y-linear
+// part of f (rows other than a; the a-column and the a-row go through the
+// features) and its a-derivative; the features at a; and the functional sums
+// of sum_rows w f_row(a, Y) with their a-derivatives.
+template <int NS>
+DEB_OUTLINE void acol_pass(Ctx &cx, const uint32_t (&rd)[NS], dual a, double invtau, const double (&w_in)[NS],
+                           const double (&Y_in)[NS], dual (&out_io)[NS], dual (&F_out)[FT_N],
+                           RowSumsT<dual> &rs) {
+  using T = dual;
+  const int lane = lane_id();
   double w[NS], Y[NS];
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
@@ -728,12 +682,25 @@ DEB_OUTLINE void row_sums_t(Ctx &cx, const uint32_t (&rd)[NS], T a, double invta
     Y[s] = Y_in[s];
   }
   const Bg<T> b = background<T>(cx, a);
-  T part[7];
+  T F[FT_N];
+  features_of<T>(cx, a, b, F);
+#pragma unroll
+  for (int m = 0; m < FT_N; ++m) F_out[m] = F[m];
+  T out[NS], part[7], cc[4][NS];
 #pragma unroll
   for (int i = 0; i < 7; ++i) part[i] = tconst<T>(0.0);
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
     const Coef<T> r = row_coef_slot<T>(rd[s], b, cx, invtau, s == 0);
+    T sw = r.sub * w[s];
+    T pw = r.sup * w[s];
+    out[s] = r.diag * w[s] + t_from_next(sw) + t_from_prev(pw);
+    if (s == 0) {
+      T lw = r.leaf * w[0];
+      T l_dg = t_readlane(lw, L_DG), l_tg = t_readlane(lw, L_TG);
+      if (lane == L_TG) out[0] = out[0] + l_dg;
+      if (lane == L_DG) out[0] = out[0] + l_tg;
+    }
     part[0] = part[0] + r.ch * Y[s];
     part[1] = part[1] + r.ceta * Y[s];
     part[2] = part[2] + r.cpi * Y[s];
@@ -741,9 +708,17 @@ DEB_OUTLINE void row_sums_t(Ctx &cx, const uint32_t (&rd)[NS], T a, double invta
     part[4] = part[4] + r.uh * w[s];
     part[5] = part[5] + r.ueta * w[s];
     part[6] = part[6] + r.upi * w[s];
+    cc[0][s] = r.ch;
+    cc[1][s] = r.ceta;
+    cc[2][s] = r.cpi;
+    cc[3][s] = r.c1;
   }
   t_wave_sum<T, 7>(part);
   rs = RowSumsT<T>{part[0], part[1], part[2], part[3], part[4], part[5], part[6]};
+  const double wahp = readlane(w[0], L_AHP);
+#pragma unroll
+  for (int s = 0; s < NS; ++s)
+    out_io[s] = out[s] + cc[0][s] * part[4] + cc[1][s] * part[5] + cc[2][s] * part[6] + cc[3][s] * wahp;
 }
 
 // Second order along one direction per lane: with P_m = dS/dF_m and
@@ -1010,16 +985,15 @@ __device__ __forceinline__ void rodas5_step_vjp(Mode<NS> &m, Ctx &cx, double t0,
     dual ad;
     ad.v = a0;
     ad.d = 1.0;
-    dual gy[NS];
-    tapply<NS, dual>(cx, m.rd, ad, it0, rho, gy);
+    // rho^T A'(a) (the derivative of the transposed apply), and for
+    // S = rho^T f(a, y0) the features and functional sums in dual arithmetic
+    dual gy[NS], fvd[1][FT_N], fbd[1][FT_N];
+    RowSumsT<dual> rsd[1];
+    acol_pass<NS>(cx, m.rd, ad, it0, rho, y0, gy, fvd[0], rsd[0]);
 #pragma unroll
     for (int s = 0; s < NS; ++s) y0b[s] += gy[s].d;
-    // S = rho^T f(a, y0): feature partials P and their a-derivative Q in one
-    // pass in dual arithmetic along a, then the features' second derivatives
-    dual fvd[1][FT_N], fbd[1][FT_N];
-    features_call<dual>(cx, ad, fvd[0]);
-    RowSumsT<dual> rsd[1];
-    row_sums_t<NS, dual>(cx, m.rd, ad, it0, rho, y0, rsd[0]);
+    // feature partials P and their a-derivative Q in one pass in dual
+    // arithmetic along a, then the features' second derivatives
     double Yd[1][NS], wd[1][NS];
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
