@@ -261,9 +261,10 @@ using RowSums = RowSumsT<double>;
 // With DFA, also the features' derivatives along a (dfa_out), from a dual
 // evaluation beside the plain one (the plain values feed the rows unchanged).
 template <int NS, bool TAPPLY, bool DFA = false>
-DEB_OUTLINE RowSums coef_pass(Ctx &cx, const uint32_t (&rd)[NS], double a, double invtau,
-                              const double (&w_in)[NS], const double (&Y_in)[NS], double (&out)[NS],
-                              double (&fv_out)[FT_N], double *dfa_out = nullptr) {
+__device__ __forceinline__ RowSums coef_pass_body(Ctx &cx, const uint32_t (&rd)[NS], double a, double invtau,
+                                                  const double (&w_in)[NS], const double (&Y_in)[NS],
+                                                  double (&out)[NS], double (&fv_out)[FT_N],
+                                                  double *dfa_out = nullptr) {
   const int lane = lane_id();
   double w[NS], Y[NS];
 #pragma unroll
@@ -320,6 +321,12 @@ DEB_OUTLINE RowSums coef_pass(Ctx &cx, const uint32_t (&rd)[NS], double a, doubl
     for (int s = 0; s < NS; ++s) out[s] = o[s] + cc[0][s] * rs.sh + cc[1][s] * rs.se + cc[2][s] * rs.sp + cc[3][s] * wahp;
   }
   return rs;
+}
+template <int NS, bool TAPPLY, bool DFA = false>
+DEB_OUTLINE RowSums coef_pass(Ctx &cx, const uint32_t (&rd)[NS], double a, double invtau,
+                              const double (&w_in)[NS], const double (&Y_in)[NS], double (&out)[NS],
+                              double (&fv_out)[FT_N], double *dfa_out = nullptr) {
+  return coef_pass_body<NS, TAPPLY, DFA>(cx, rd, a, invtau, w_in, Y_in, out, fv_out, dfa_out);
 }
 
 // Partials of the rows' sum_fields mul[q][f] coef_f with respect to the
@@ -518,9 +525,10 @@ __device__ __forceinline__ void slot_muls(int s, double ys, double ws, double ym
 // sum_rows w[q] f_row(Y[q]) (slot_muls, formed slot by slot here; rs[q] the
 // functionals' values): K contractions in one pass of the divergent row switch
 template <int NS, int K, class T = double>
-DEB_OUTLINE void feature_partials(const Ctx &cx, const uint32_t (&rd)[NS], const T (&fv_in)[K][FT_N],
-                                  const double (&Y_in)[K][NS], const double (&w_in)[K][NS],
-                                  const RowSumsT<T> (&rs_in)[K], T (&fb)[K][FT_N]) {
+__device__ __forceinline__ void feature_partials_body(const Ctx &cx, const uint32_t (&rd)[NS],
+                                                      const T (&fv_in)[K][FT_N], const double (&Y_in)[K][NS],
+                                                      const double (&w_in)[K][NS], const RowSumsT<T> (&rs_in)[K],
+                                                      T (&fb)[K][FT_N]) {
   const int lane = lane_id();
   // Private copies: fb may alias the inputs as far as the compiler knows, so
   // accumulating through the references would reload every operand after each
@@ -596,6 +604,12 @@ DEB_OUTLINE void feature_partials(const Ctx &cx, const uint32_t (&rd)[NS], const
   for (int q = 0; q < K; ++q)
 #pragma unroll
     for (int m = 0; m < FT_N; ++m) fb[q][m] += acc[q][m];
+}
+template <int NS, int K, class T = double>
+DEB_OUTLINE void feature_partials(const Ctx &cx, const uint32_t (&rd)[NS], const T (&fv_in)[K][FT_N],
+                                  const double (&Y_in)[K][NS], const double (&w_in)[K][NS],
+                                  const RowSumsT<T> (&rs_in)[K], T (&fb)[K][FT_N]) {
+  feature_partials_body<NS, K, T>(cx, rd, fv_in, Y_in, w_in, rs_in, fb);
 }
 
 // One tangent direction per lane: lane & 7 == 0 carries d/da (component 0 of
@@ -766,6 +780,53 @@ __device__ __forceinline__ RowSums jac_sums(const Mode<NS> &m, const double (&Y)
   return RowSums{part[0], part[1], part[2], part[3], part[4], part[5], part[6]};
 }
 
+// The row work of one reverse stage as a single call: the transposed row pass
+// at the stage point (ub = J(t_i, u_i)^T rb, the stage's features and their
+// a-derivative dfa), the row sums of the J(t0, y0) k_i term, and the feature
+// partials of both contractions -- the stage's into fb0, the J-term's (at the
+// features fvJ of a0, multipliers without the a-row) accumulated into fbJ.
+template <int NS>
+DEB_OUTLINE void stage_rows(Ctx &cx, const Mode<NS> &m, double ai, double iti, const double (&rb_in)[NS],
+                            const double (&u_in)[NS], const double (&ki_in)[NS], const double (&fvJ_in)[FT_N],
+                            double (&ub_out)[NS], double (&fb0_out)[FT_N], double (&fbJ_io)[FT_N],
+                            double (&dfa_out)[FT_N]) {
+  const int lane = lane_id();
+  double rb[NS], u[NS], ki[NS], ub[NS], dfa[FT_N];
+  double fvs[2][FT_N], fbs[2][FT_N];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    rb[s] = rb_in[s];
+    u[s] = u_in[s];
+    ki[s] = ki_in[s];
+  }
+#pragma unroll
+  for (int q = 0; q < FT_N; ++q) {
+    fvs[1][q] = fvJ_in[q];
+    fbs[0][q] = 0.0;
+    fbs[1][q] = fbJ_io[q];
+  }
+  RowSums rsq[2];
+  rsq[0] = coef_pass_body<NS, true, true>(cx, m.rd, ai, iti, rb, u, ub, fvs[0], dfa);
+  double Yq[2][NS], wq[2][NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    Yq[0][s] = u[s];
+    wq[0][s] = rb[s];
+    Yq[1][s] = ki[s];
+    wq[1][s] = (s == 0 && lane == L_A) ? 0.0 : rb[s];
+  }
+  rsq[1] = jac_sums<NS>(m, ki, wq[1]);
+  feature_partials_body<NS, 2>(cx, m.rd, fvs, Yq, wq, rsq, fbs);
+#pragma unroll
+  for (int s = 0; s < NS; ++s) ub_out[s] = ub[s];
+#pragma unroll
+  for (int q = 0; q < FT_N; ++q) {
+    fb0_out[q] = fbs[0][q];
+    fbJ_io[q] = fbs[1][q];
+    dfa_out[q] = dfa[q];
+  }
+}
+
 // per-lane multiplier of the explicit tau dependence (hierarchy truncations):
 // d f_row / d tau = trunc_row / tau^2 * y_row with trunc_row = l + 1
 template <int NS>
@@ -844,14 +905,14 @@ __device__ __forceinline__ void rodas5_step_vjp(Mode<NS> &m, Ctx &cx, double t0,
 #pragma unroll
   for (int j = 0; j < MAXDIR; ++j) th_acc[j] = 0.0;
   double a0b = 0.0, t0b = 0.0, hb = 0.0;
-  // Feature partials, one row pass per stage with two contractions: f(t_i, u_i)
-  // at the stage's features (fvs[0], fbs[0], contracted every stage) and the
-  // J(t0, y0) k_i term at the features of a0 (fvs[1]; its partials fbs[1]
-  // accumulate over the stages and are contracted once).
-  double fvs[2][FT_N], fbs[2][FT_N];
-  feature_values(cx, a0, fvs[1]);
+  // Feature partials, one row pass per stage (stage_rows) with two
+  // contractions: f(t_i, u_i) at the stage's features (contracted every
+  // stage) and the J(t0, y0) k_i term at the features fvJ of a0 (its partials
+  // fbJs accumulate over the stages and are contracted once).
+  double fvJ[FT_N], fbJs[FT_N];
+  feature_values(cx, a0, fvJ);
 #pragma unroll
-  for (int q = 0; q < FT_N; ++q) fbs[1][q] = 0.0;
+  for (int q = 0; q < FT_N; ++q) fbJs[q] = 0.0;
 #pragma nounroll
   for (int i = 7; i >= 0; --i) {
     double rb[NS], ki[NS];
@@ -914,24 +975,8 @@ __device__ __forceinline__ void rodas5_step_vjp(Mode<NS> &m, Ctx &cx, double t0,
     double ub[NS];
     {
       const double ai = readlane(u[0], L_A);
-      double Yq[2][NS], wq[2][NS];
-      RowSums rsq[2];
-      double dfa[FT_N];
-      rsq[0] = coef_pass<NS, true, true>(cx, m.rd, ai, iti, rb, u, ub, fvs[0], dfa);
-#pragma unroll
-      for (int s = 0; s < NS; ++s) {
-        Yq[0][s] = u[s];
-        wq[0][s] = rb[s];
-        Yq[1][s] = ki[s];
-        wq[1][s] = (s == 0 && lane == L_A) ? 0.0 : rb[s];
-      }
-      rsq[1] = jac_sums<NS>(m, ki, wq[1]);
-#pragma unroll
-      for (int q = 0; q < FT_N; ++q) fbs[0][q] = 0.0;
-      feature_partials<NS, 2>(cx, m.rd, fvs, Yq, wq, rsq, fbs);
-      double fb[FT_N];
-#pragma unroll
-      for (int q = 0; q < FT_N; ++q) fb[q] = fbs[0][q];
+      double dfa[FT_N], fb[FT_N];
+      stage_rows<NS>(cx, m, ai, iti, rb, u, ki, fvJ, ub, fb, fbJs, dfa);
       wave_sum<FT_N>(fb);
       // the a-direction now (it feeds the earlier stages); the input-tangent
       // directions of all stages in one pass after the loop
@@ -971,7 +1016,7 @@ __device__ __forceinline__ void rodas5_step_vjp(Mode<NS> &m, Ctx &cx, double t0,
   {
     double fbJ[FT_N];
 #pragma unroll
-    for (int q = 0; q < FT_N; ++q) fbJ[q] = fbs[1][q];
+    for (int q = 0; q < FT_N; ++q) fbJ[q] = fbJs[q];
     wave_sum<FT_N>(fbJ);
     const double g = lane_contract(cx, a0, fbJ);
     a0b += readlane(g, 0);
