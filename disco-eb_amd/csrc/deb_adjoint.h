@@ -679,14 +679,14 @@ DEB_OUTLINE double lane_contract_stages(const Ctx &cx, const lds_double *cst) {
 }
 
 // The a-column pass of the reverse step in dual arithmetic along a, with one
-// background evaluation for its three parts: out = A(a)^T w for the y-linear
-// part of f (rows other than a; the a-column and the a-row go through the
-// features) and its a-derivative; the features at a; and the functional sums
-// of sum_rows w f_row(a, Y) with their a-derivatives.
+// background evaluation for all of it: out = A(a)^T w for the y-linear part of
+// f (rows other than a; the a-column and the a-row go through the features)
+// and its a-derivative; then, for S = sum_rows w f_row(a, Y), the functional
+// sums and the per-lane feature partials P (fb.v) with their a-derivatives Q
+// (fb.d), at the features of a.
 template <int NS>
 DEB_OUTLINE void acol_pass(Ctx &cx, const uint32_t (&rd)[NS], dual a, double invtau, const double (&w_in)[NS],
-                           const double (&Y_in)[NS], dual (&out_io)[NS], dual (&F_out)[FT_N],
-                           RowSumsT<dual> &rs) {
+                           const double (&Y_in)[NS], dual (&out_io)[NS], dual (&fb_out)[FT_N]) {
   using T = dual;
   const int lane = lane_id();
   double w[NS], Y[NS];
@@ -696,10 +696,8 @@ DEB_OUTLINE void acol_pass(Ctx &cx, const uint32_t (&rd)[NS], dual a, double inv
     Y[s] = Y_in[s];
   }
   const Bg<T> b = background<T>(cx, a);
-  T F[FT_N];
-  features_of<T>(cx, a, b, F);
-#pragma unroll
-  for (int m = 0; m < FT_N; ++m) F_out[m] = F[m];
+  T F[1][FT_N];
+  features_of<T>(cx, a, b, F[0]);
   T out[NS], part[7], cc[4][NS];
 #pragma unroll
   for (int i = 0; i < 7; ++i) part[i] = tconst<T>(0.0);
@@ -728,11 +726,23 @@ DEB_OUTLINE void acol_pass(Ctx &cx, const uint32_t (&rd)[NS], dual a, double inv
     cc[3][s] = r.c1;
   }
   t_wave_sum<T, 7>(part);
-  rs = RowSumsT<T>{part[0], part[1], part[2], part[3], part[4], part[5], part[6]};
   const double wahp = readlane(w[0], L_AHP);
 #pragma unroll
   for (int s = 0; s < NS; ++s)
     out_io[s] = out[s] + cc[0][s] * part[4] + cc[1][s] * part[5] + cc[2][s] * part[6] + cc[3][s] * wahp;
+  const RowSumsT<T> rs[1] = {RowSumsT<T>{part[0], part[1], part[2], part[3], part[4], part[5], part[6]}};
+  double Yd[1][NS], wd[1][NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    Yd[0][s] = Y[s];
+    wd[0][s] = w[s];
+  }
+  T fb[1][FT_N];
+#pragma unroll
+  for (int q = 0; q < FT_N; ++q) fb[0][q] = tconst<T>(0.0);
+  feature_partials_body<NS, 1, T>(cx, rd, F, Yd, wd, rs, fb);
+#pragma unroll
+  for (int q = 0; q < FT_N; ++q) fb_out[q] = fb[0][q];
 }
 
 // Second order along one direction per lane: with P_m = dS/dF_m and
@@ -1031,28 +1041,18 @@ __device__ __forceinline__ void rodas5_step_vjp(Mode<NS> &m, Ctx &cx, double t0,
     ad.v = a0;
     ad.d = 1.0;
     // rho^T A'(a) (the derivative of the transposed apply), and for
-    // S = rho^T f(a, y0) the features and functional sums in dual arithmetic
-    dual gy[NS], fvd[1][FT_N], fbd[1][FT_N];
-    RowSumsT<dual> rsd[1];
-    acol_pass<NS>(cx, m.rd, ad, it0, rho, y0, gy, fvd[0], rsd[0]);
+    // S = rho^T f(a, y0) the per-lane feature partials in dual arithmetic
+    dual gy[NS], fbd[FT_N];
+    acol_pass<NS>(cx, m.rd, ad, it0, rho, y0, gy, fbd);
 #pragma unroll
     for (int s = 0; s < NS; ++s) y0b[s] += gy[s].d;
-    // feature partials P and their a-derivative Q in one pass in dual
-    // arithmetic along a, then the features' second derivatives
-    double Yd[1][NS], wd[1][NS];
-#pragma unroll
-    for (int s = 0; s < NS; ++s) {
-      Yd[0][s] = y0[s];
-      wd[0][s] = rho[s];
-    }
-#pragma unroll
-    for (int q = 0; q < FT_N; ++q) fbd[0][q] = tconst<dual>(0.0);
-    feature_partials<NS, 1, dual>(cx, m.rd, fvd, Yd, wd, rsd, fbd);
+    // the feature partials P and their a-derivative Q, summed over the wave,
+    // then the features' second derivatives
     double PQ[2 * FT_N];
 #pragma unroll
     for (int q = 0; q < FT_N; ++q) {
-      PQ[q] = fbd[0][q].v;
-      PQ[FT_N + q] = fbd[0][q].d;
+      PQ[q] = fbd[q].v;
+      PQ[FT_N + q] = fbd[q].d;
     }
     wave_sum<2 * FT_N>(PQ);
     double P[FT_N], Q[FT_N];
