@@ -1022,19 +1022,12 @@ __device__ __forceinline__ void rodas5_step_vjp(Mode<NS> &m, Ctx &cx, double t0,
       }
     }
   }
-  // the J(t0, y0) k_i terms, contracted with the feature tangents at a0
-  {
-    double fbJ[FT_N];
+  // the J(t0, y0) k_i terms' partials, contracted with the feature tangents
+  // at a0 together with the a-column's below (first order: added to Q)
+  double fbJ[FT_N];
 #pragma unroll
-    for (int q = 0; q < FT_N; ++q) fbJ[q] = fbJs[q];
-    wave_sum<FT_N>(fbJ);
-    const double g = lane_contract(cx, a0, fbJ);
-    a0b += readlane(g, 0);
-    double tl[MAXDIR];
-    lane_gather(g, cx.ndir, tl);
-#pragma unroll
-    for (int j = 0; j < MAXDIR; ++j) th_acc[j] += tl[j];
-  }
+  for (int q = 0; q < FT_N; ++q) fbJ[q] = fbJs[q];
+  wave_sum<FT_N>(fbJ);
   // a-column of J(t0, y0): rho^T df/da, differentiated along y0, a0 and theta
   {
     dual ad;
@@ -1059,7 +1052,7 @@ __device__ __forceinline__ void rodas5_step_vjp(Mode<NS> &m, Ctx &cx, double t0,
 #pragma unroll
     for (int q = 0; q < FT_N; ++q) {
       P[q] = PQ[q];
-      Q[q] = PQ[FT_N + q];
+      Q[q] = PQ[FT_N + q] + fbJ[q];
     }
     const double g = lane_contract2(cx, a0, P, Q);
     a0b += readlane(g, 0);
