@@ -986,7 +986,35 @@ __device__ __forceinline__ void rodas5_step_vjp(Mode<NS> &m, Ctx &cx, double t0,
     {
       const double ai = readlane(u[0], L_A);
       double dfa[FT_N], fb[FT_N];
-      stage_rows<NS>(cx, m, ai, iti, rb, u, ki, fvJ, ub, fb, fbJs, dfa);
+      if constexpr (NS <= 4) {
+        stage_rows<NS>(cx, m, ai, iti, rb, u, ki, fvJ, ub, fb, fbJs, dfa);
+      } else {
+        // five slots: the same work as separate calls (as one call the
+        // reverse launch of the lmax-50 workload measured 14 % slower)
+        double fvs[2][FT_N], fbs[2][FT_N], Yq[2][NS], wq[2][NS];
+        RowSums rsq[2];
+        rsq[0] = coef_pass<NS, true, true>(cx, m.rd, ai, iti, rb, u, ub, fvs[0], dfa);
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+          Yq[0][s] = u[s];
+          wq[0][s] = rb[s];
+          Yq[1][s] = ki[s];
+          wq[1][s] = (s == 0 && lane == L_A) ? 0.0 : rb[s];
+        }
+        rsq[1] = jac_sums<NS>(m, ki, wq[1]);
+#pragma unroll
+        for (int q = 0; q < FT_N; ++q) {
+          fvs[1][q] = fvJ[q];
+          fbs[0][q] = 0.0;
+          fbs[1][q] = fbJs[q];
+        }
+        feature_partials<NS, 2>(cx, m.rd, fvs, Yq, wq, rsq, fbs);
+#pragma unroll
+        for (int q = 0; q < FT_N; ++q) {
+          fb[q] = fbs[0][q];
+          fbJs[q] = fbs[1][q];
+        }
+      }
       wave_sum<FT_N>(fb);
       // the a-direction now (it feeds the earlier stages); the input-tangent
       // directions of all stages in one pass after the loop

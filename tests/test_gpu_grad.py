@@ -27,9 +27,12 @@ def _cfg(**kw):
 
 
 def _layout(name):
-    """The small test layout or the production one (lmax 32, lmaxnu 17, nq 5)."""
+    """The small test layout, the production one (lmax 32, lmaxnu 17, nq 5) or
+    the lmax-50 one of the loss workload (five lane slots)."""
     if name == "small":
         return _cfg()
+    if name == "lmax50":
+        return packing.make_config(lmaxg=50, lmaxgp=50, lmaxr=50, lmaxnu=17, nqmax=5)
     return packing.make_config(lmaxg=32, lmaxgp=32, lmaxr=32, lmaxnu=17, nqmax=5)
 
 
@@ -124,7 +127,7 @@ def test_rhs_vjp_dot_product(grad_inputs, gpu, cfgname, era):
 
 
 @pytest.mark.parametrize("era", ["early", "late"])
-@pytest.mark.parametrize("cfgname", ["small", "lmax32-nq5"])
+@pytest.mark.parametrize("cfgname", ["small", "lmax32-nq5", "lmax50"])
 def test_step_vjp_dot_product(grad_inputs, gpu, cfgname, era):
     g = grad_inputs
     cfg = _layout(cfgname)
