@@ -122,8 +122,8 @@ __device__ __forceinline__ void bsolveT(const Mode<NS> &m, double (&x)[NS]) {
   const int lane = lane_id();
   // leaf back-substitution x_dg = (r_dg - wdg x_tg) ddg
   const double xdg = readlane(x[0], L_DG);
-  double rdgb = m.ddg * xdg;
-  if (lane == L_TG) x[0] -= m.wdg * m.ddg * xdg;
+  double rdgb = m.u->ddg * xdg;
+  if (lane == L_TG) x[0] -= m.u->wdg * m.u->ddg * xdg;
   if (lane == L_DG) x[0] = 0.0;
 #pragma unroll
   for (int j = 0; j < NS; ++j) x[j] *= m.binv[j];
@@ -152,7 +152,7 @@ __device__ __forceinline__ void bsolveT(const Mode<NS> &m, double (&x)[NS]) {
     }
   }
   // leaf pre-elimination x_tg -= leafm r_dg
-  rdgb -= m.leafm * readlane(x[0], L_TG);
+  rdgb -= m.u->leafm * readlane(x[0], L_TG);
   if (lane == L_DG) x[0] += rdgb;
 }
 
@@ -167,9 +167,9 @@ DEB_OUTLINE void wsolveT(const Mode<NS> &m, double (&xio)[NS]) {
   for (int s = 0; s < NS; ++s) x[s] = xio[s];
   const double kba = readlane(x[0], L_A), kb1 = readlane(x[0], L_AHP);
   if (lane == L_A || lane == L_AHP) x[0] = 0.0;
-  const double r1b = m.gh * kb1;
-  double kab = kba + m.gh * m.J1a * kb1;
-  const double c1kb = m.gh * kb1;
+  const double r1b = m.u->gh * kb1;
+  double kab = kba + m.u->gh * m.u->J1a * kb1;
+  const double c1kb = m.u->gh * kb1;
   double part[3] = {0.0, 0.0, 0.0};
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
@@ -180,10 +180,10 @@ DEB_OUTLINE void wsolveT(const Mode<NS> &m, double (&xio)[NS]) {
   wave_sum<3>(part);
   double tt[3];
 #pragma unroll
-  for (int r = 0; r < 3; ++r) tt[r] = m.c1Z[r] * c1kb + part[r];
+  for (int r = 0; r < 3; ++r) tt[r] = m.u->c1Z[r] * c1kb + part[r];
   double gb[3];
 #pragma unroll
-  for (int c = 0; c < 3; ++c) gb[c] = m.Sinv[0][c] * tt[0] + m.Sinv[1][c] * tt[1] + m.Sinv[2][c] * tt[2];
+  for (int c = 0; c < 3; ++c) gb[c] = m.u->Sinv[0][c] * tt[0] + m.u->Sinv[1][c] * tt[1] + m.u->Sinv[2][c] * tt[2];
 #pragma unroll
   for (int s = 0; s < NS; ++s)
     x[s] += m.Cf[0][s] * gb[0] + m.Cf[1][s] * gb[1] + m.Cf[2][s] * gb[2] + m.c1[s] * c1kb;
@@ -192,7 +192,7 @@ DEB_OUTLINE void wsolveT(const Mode<NS> &m, double (&xio)[NS]) {
 #pragma unroll
   for (int s = 0; s < NS; ++s) pj += m.jcol[s] * x[s];
   kab += wave_sum1(pj);
-  if (lane == L_A) x[0] = m.inv_kaden * kab;
+  if (lane == L_A) x[0] = m.u->inv_kaden * kab;
   if (lane == L_AHP) x[0] = r1b;
 #pragma unroll
   for (int s = 0; s < NS; ++s) xio[s] = x[s];
@@ -1322,8 +1322,10 @@ __device__ __forceinline__ void adjoint_mode(const Params<NS> &P, const AdjArgs 
   cx.ndir = A.ndir;
   cx.dcos = A.dcos + (long long)ci * A.ndir * COSMO_DOUBLES;
   cx.dtab = A.dtab + (long long)ci * A.ndir * P.table_stride;
+  __shared__ ModeU mu;
   Mode<NS> m;
   m.pcr = pcr;
+  m.u = (lds_mode_u *)&mu;
 #pragma unroll
   for (int s = 0; s < NS; ++s) m.rd[s] = P.rowdesc[s][lane];
   m.nlev = mode_levels(P.cfg, P.nq);

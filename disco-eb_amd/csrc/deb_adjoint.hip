@@ -151,8 +151,10 @@ __global__ void __launch_bounds__(WAVE) deb_step_vjp_kernel(const Params<NS> P, 
   const int lane = lane_id();
   Ctx cx;
   hook_ctx<NS>(cx, P, H, item);
+  __shared__ ModeU mu;
   Mode<NS> m;
   m.pcr = (lds_double *)pcr;
+  m.u = (lds_mode_u *)&mu;
   m.nlev = mode_levels(P.cfg, P.nq);
   double y[NS], yb[NS];
   const double *ys = H.y + (long long)item * P.n, *ws = H.wbar + (long long)item * P.n;

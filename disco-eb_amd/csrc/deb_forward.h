@@ -295,8 +295,10 @@ __device__ __forceinline__ void integrate_mode(const Params<NS> &P, int ci, int 
   Ctx cx;
   const double k = P.kmodes[(int64_t)ci * P.nk + ki];
   make_ctx<NS>(cx, P, ci, k);
+  __shared__ ModeU mu;  // the wave-uniform step factors
   Mode<NS> m;
   m.pcr = pcr;
+  m.u = (lds_mode_u *)&mu;
 #pragma unroll
   for (int s = 0; s < NS; ++s) m.rd[s] = P.rowdesc[s][lane];
   m.nlev = mode_levels(P.cfg, P.nq);
@@ -411,8 +413,10 @@ __device__ __forceinline__ void integrate_batch(const Params<NS> &P, int ci, int
   const int64_t mode0 = (int64_t)ci * P.nk + kb;
   Ctx cx;
   make_ctx<NS>(cx, P, ci, P.kmodes[mode0]);
+  __shared__ ModeU mu;  // the wave-uniform step factors
   Mode<NS> m;
   m.pcr = pcr;
+  m.u = (lds_mode_u *)&mu;
 #pragma unroll
   for (int s = 0; s < NS; ++s) m.rd[s] = P.rowdesc[s][lane];
   m.nlev = mode_levels(P.cfg, P.nq);

@@ -128,8 +128,10 @@ __global__ void __launch_bounds__(WAVE) deb_step_kernel(const Params<NS> P, cons
   const int lane = lane_id();
   Ctx cx;
   make_ctx<NS>(cx, P, H.ci[item], H.kk[item]);
+  __shared__ ModeU mu;
   Mode<NS> m;
   m.pcr = (lds_double *)pcr;
+  m.u = (lds_mode_u *)&mu;
   {
     int maxlen = P.cfg.lmaxg + 2;
     maxlen = max(maxlen, P.cfg.lmaxgp + 1);

@@ -14,6 +14,17 @@
 
 namespace deb {
 
+// wave-uniform step factors of Mode
+struct ModeU {
+  double gh;         // gamma * h
+  double inv_kaden;  // 1 / (1/(gamma h) - g')
+  double J1a;
+  double leafm, wdg, ddg;  // leaf elimination: W_tg,dg / d_dg, W_dg,tg, 1/d_dg
+  double Sinv[3][3];
+  double c1Z[3];
+};
+typedef DEB_LDS ModeU lds_mode_u;
+
 template <int NS>
 struct Mode {
   uint32_t rd[NS];
@@ -31,13 +42,9 @@ struct Mode {
   double c1[NS];
   double U[3][NS];   // u_h, u_eta, u_pi (where h', eta', Pi enter each row)
   double jcol[NS];   // df/da at the step start
-  // uniform
-  double gh;         // gamma * h
-  double inv_kaden;  // 1 / (1/(gamma h) - g')
-  double J1a;
-  double leafm, wdg, ddg;  // leaf elimination: W_tg,dg / d_dg, W_dg,tg, 1/d_dg
-  double Sinv[3][3];
-  double c1Z[3];
+  // the wave-uniform factors, in the wave's LDS (one copy per wave rather than
+  // one per lane in registers or scratch)
+  lds_mode_u *u;
   int nlev;
 };
 
@@ -130,7 +137,7 @@ template <int NS>
 __device__ __forceinline__ void bsolve(const Mode<NS> &m, double (&x)[NS]) {
   const int lane = lane_id();
   const double rdg = readlane(x[0], L_DG);
-  if (lane == L_TG) x[0] -= m.leafm * rdg;
+  if (lane == L_TG) x[0] -= m.u->leafm * rdg;
 #pragma unroll
   for (int L = 0; L < PCR_LEVELS; ++L) {
     if (L >= m.nlev) break;
@@ -154,7 +161,7 @@ __device__ __forceinline__ void bsolve(const Mode<NS> &m, double (&x)[NS]) {
 #pragma unroll
   for (int j = 0; j < NS; ++j) x[j] *= m.binv[j];
   const double xtg = readlane(x[0], L_TG);
-  if (lane == L_DG) x[0] = (rdg - m.wdg * xtg) * m.ddg;
+  if (lane == L_DG) x[0] = (rdg - m.u->wdg * xtg) * m.u->ddg;
 }
 
 // factorisation of W at (tau0, y0) for step size h
@@ -165,11 +172,11 @@ __device__ __forceinline__ void factor(Mode<NS> &m, Ctx &cx, double tau0, const 
   dual fd[NS];
   Coef<dual> co[NS];
   eval_rhs<NS, dual>(cx, m.rd, tau0, y0, fd, &co);
-  m.gh = gamma * h;
-  const double igh = 1.0 / m.gh;
+  m.u->gh = gamma * h;
+  const double igh = 1.0 / m.u->gh;
   double gprime = readlane(fd[0].d, L_A);
-  m.J1a = readlane(fd[0].d, L_AHP);
-  m.inv_kaden = 1.0 / (igh - gprime);
+  m.u->J1a = readlane(fd[0].d, L_AHP);
+  m.u->inv_kaden = 1.0 / (igh - gprime);
   double A[NS], Bd[NS], Cc[NS];
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
@@ -190,10 +197,10 @@ __device__ __forceinline__ void factor(Mode<NS> &m, Ctx &cx, double tau0, const 
   // leaf: delta_gamma (lane L_DG) couples only to theta_gamma (lane L_TG)
   const double w_dg_tg = -readlane(co[0].leaf.v, L_DG);
   const double w_tg_dg = -readlane(co[0].leaf.v, L_TG);
-  m.ddg = m.gh;  // 1 / d_dg with d_dg = 1/(gamma h)
-  m.wdg = w_dg_tg;
-  m.leafm = w_tg_dg * m.gh;
-  if (lane == L_TG) Bd[0] -= w_tg_dg * w_dg_tg * m.gh;
+  m.u->ddg = m.u->gh;  // 1 / d_dg with d_dg = 1/(gamma h)
+  m.u->wdg = w_dg_tg;
+  m.u->leafm = w_tg_dg * m.u->gh;
+  if (lane == L_TG) Bd[0] -= w_tg_dg * w_dg_tg * m.u->gh;
   // PCR factorisation; the dark-energy 2x2 block (lanes L_DQ, L_TQ of slot 0)
   // is eliminated at level 0 by cross-multiplication (its theta_Q pivot can
   // vanish), every other row by the usual reduction, which multiplies by the
@@ -275,18 +282,18 @@ __device__ __forceinline__ void factor(Mode<NS> &m, Ctx &cx, double tau0, const 
                      S[0][1] * (S[1][0] * S[2][2] - S[1][2] * S[2][0]) +
                      S[0][2] * (S[1][0] * S[2][1] - S[1][1] * S[2][0]);
   const double id = 1.0 / det;
-  m.Sinv[0][0] = (S[1][1] * S[2][2] - S[1][2] * S[2][1]) * id;
-  m.Sinv[0][1] = (S[0][2] * S[2][1] - S[0][1] * S[2][2]) * id;
-  m.Sinv[0][2] = (S[0][1] * S[1][2] - S[0][2] * S[1][1]) * id;
-  m.Sinv[1][0] = (S[1][2] * S[2][0] - S[1][0] * S[2][2]) * id;
-  m.Sinv[1][1] = (S[0][0] * S[2][2] - S[0][2] * S[2][0]) * id;
-  m.Sinv[1][2] = (S[0][2] * S[1][0] - S[0][0] * S[1][2]) * id;
-  m.Sinv[2][0] = (S[1][0] * S[2][1] - S[1][1] * S[2][0]) * id;
-  m.Sinv[2][1] = (S[0][1] * S[2][0] - S[0][0] * S[2][1]) * id;
-  m.Sinv[2][2] = (S[0][0] * S[1][1] - S[0][1] * S[1][0]) * id;
-  m.c1Z[0] = part[9];
-  m.c1Z[1] = part[10];
-  m.c1Z[2] = part[11];
+  m.u->Sinv[0][0] = (S[1][1] * S[2][2] - S[1][2] * S[2][1]) * id;
+  m.u->Sinv[0][1] = (S[0][2] * S[2][1] - S[0][1] * S[2][2]) * id;
+  m.u->Sinv[0][2] = (S[0][1] * S[1][2] - S[0][2] * S[1][1]) * id;
+  m.u->Sinv[1][0] = (S[1][2] * S[2][0] - S[1][0] * S[2][2]) * id;
+  m.u->Sinv[1][1] = (S[0][0] * S[2][2] - S[0][2] * S[2][0]) * id;
+  m.u->Sinv[1][2] = (S[0][2] * S[1][0] - S[0][0] * S[1][2]) * id;
+  m.u->Sinv[2][0] = (S[1][0] * S[2][1] - S[1][1] * S[2][0]) * id;
+  m.u->Sinv[2][1] = (S[0][1] * S[2][0] - S[0][0] * S[2][1]) * id;
+  m.u->Sinv[2][2] = (S[0][0] * S[1][1] - S[0][1] * S[1][0]) * id;
+  m.u->c1Z[0] = part[9];
+  m.u->c1Z[1] = part[10];
+  m.u->c1Z[2] = part[11];
 }
 
 // k = W^{-1} r (r is overwritten by k)
@@ -295,7 +302,7 @@ __device__ __forceinline__ void wsolve(const Mode<NS> &m, double (&r)[NS]) {
   const int lane = lane_id();
   const double ra = readlane(r[0], L_A);
   const double r1 = readlane(r[0], L_AHP);
-  const double ka = ra * m.inv_kaden;
+  const double ka = ra * m.u->inv_kaden;
 #pragma unroll
   for (int s = 0; s < NS; ++s) r[s] += m.jcol[s] * ka;
   if (lane == L_A || lane == L_AHP) r[0] = 0.0;
@@ -311,11 +318,11 @@ __device__ __forceinline__ void wsolve(const Mode<NS> &m, double (&r)[NS]) {
   wave_sum<4>(g);
   double t[3];
 #pragma unroll
-  for (int i = 0; i < 3; ++i) t[i] = m.Sinv[i][0] * g[0] + m.Sinv[i][1] * g[1] + m.Sinv[i][2] * g[2];
+  for (int i = 0; i < 3; ++i) t[i] = m.u->Sinv[i][0] * g[0] + m.u->Sinv[i][1] * g[1] + m.u->Sinv[i][2] * g[2];
 #pragma unroll
   for (int s = 0; s < NS; ++s) r[s] += m.Z[0][s] * t[0] + m.Z[1][s] * t[1] + m.Z[2][s] * t[2];
-  const double c1k = g[3] + m.c1Z[0] * t[0] + m.c1Z[1] * t[1] + m.c1Z[2] * t[2];
-  const double k1 = m.gh * (r1 + m.J1a * ka + c1k);
+  const double c1k = g[3] + m.u->c1Z[0] * t[0] + m.u->c1Z[1] * t[1] + m.u->c1Z[2] * t[2];
+  const double k1 = m.u->gh * (r1 + m.u->J1a * ka + c1k);
   if (lane == L_A) r[0] = ka;
   if (lane == L_AHP) r[0] = k1;
 }
