@@ -797,9 +797,9 @@ __device__ __forceinline__ RowSums jac_sums(const Mode<NS> &m, const double (&Y)
 // features fvJ of a0, multipliers without the a-row) accumulated into fbJ.
 template <int NS>
 DEB_OUTLINE void stage_rows(Ctx &cx, const Mode<NS> &m, double ai, double iti, const double (&rb_in)[NS],
-                            const double (&u_in)[NS], const double (&ki_in)[NS], const double (&fvJ_in)[FT_N],
+                            const double (&u_in)[NS], const double (&ki_in)[NS], const lds_double *fvJ_in,
                             double (&ub_out)[NS], double (&fb0_out)[FT_N], double (&fbJ_io)[FT_N],
-                            double (&dfa_out)[FT_N]) {
+                            lds_double *dfa_out) {
   const int lane = lane_id();
   double rb[NS], u[NS], ki[NS], ub[NS], dfa[FT_N];
   double fvs[2][FT_N], fbs[2][FT_N];
@@ -811,7 +811,7 @@ DEB_OUTLINE void stage_rows(Ctx &cx, const Mode<NS> &m, double ai, double iti, c
   }
 #pragma unroll
   for (int q = 0; q < FT_N; ++q) {
-    fvs[1][q] = fvJ_in[q];
+    fvs[1][q] = (double)fvJ_in[q];
     fbs[0][q] = 0.0;
     fbs[1][q] = fbJ_io[q];
   }
@@ -919,8 +919,17 @@ __device__ __forceinline__ void rodas5_step_vjp(Mode<NS> &m, Ctx &cx, double t0,
   // contractions: f(t_i, u_i) at the stage's features (contracted every
   // stage) and the J(t0, y0) k_i term at the features fvJ of a0 (its partials
   // fbJs accumulate over the stages and are contracted once).
-  double fvJ[FT_N], fbJs[FT_N];
-  feature_values(cx, a0, fvJ);
+  // the features of a0 and each stage's feature a-derivatives are wave-uniform:
+  // one copy per wave in LDS (written by every lane alike)
+  __shared__ double fvJ_[FT_N], dfa_[FT_N];
+  lds_double *const fvJ = (lds_double *)fvJ_, *const dfa = (lds_double *)dfa_;
+  double fbJs[FT_N];
+  {
+    double f[FT_N];
+    feature_values(cx, a0, f);
+#pragma unroll
+    for (int q = 0; q < FT_N; ++q) fvJ[q] = f[q];
+  }
 #pragma unroll
   for (int q = 0; q < FT_N; ++q) fbJs[q] = 0.0;
 #pragma nounroll
@@ -985,7 +994,7 @@ __device__ __forceinline__ void rodas5_step_vjp(Mode<NS> &m, Ctx &cx, double t0,
     double ub[NS];
     {
       const double ai = readlane(u[0], L_A);
-      double dfa[FT_N], fb[FT_N];
+      double fb[FT_N];
       if constexpr (NS <= 4) {
         stage_rows<NS>(cx, m, ai, iti, rb, u, ki, fvJ, ub, fb, fbJs, dfa);
       } else {
@@ -993,7 +1002,10 @@ __device__ __forceinline__ void rodas5_step_vjp(Mode<NS> &m, Ctx &cx, double t0,
         // reverse launch of the lmax-50 workload measured 14 % slower)
         double fvs[2][FT_N], fbs[2][FT_N], Yq[2][NS], wq[2][NS];
         RowSums rsq[2];
-        rsq[0] = coef_pass<NS, true, true>(cx, m.rd, ai, iti, rb, u, ub, fvs[0], dfa);
+        double dfl[FT_N];
+        rsq[0] = coef_pass<NS, true, true>(cx, m.rd, ai, iti, rb, u, ub, fvs[0], dfl);
+#pragma unroll
+        for (int q = 0; q < FT_N; ++q) dfa[q] = dfl[q];
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
           Yq[0][s] = u[s];
@@ -1004,7 +1016,7 @@ __device__ __forceinline__ void rodas5_step_vjp(Mode<NS> &m, Ctx &cx, double t0,
         rsq[1] = jac_sums<NS>(m, ki, wq[1]);
 #pragma unroll
         for (int q = 0; q < FT_N; ++q) {
-          fvs[1][q] = fvJ[q];
+          fvs[1][q] = (double)fvJ[q];
           fbs[0][q] = 0.0;
           fbs[1][q] = fbJs[q];
         }
@@ -1020,7 +1032,7 @@ __device__ __forceinline__ void rodas5_step_vjp(Mode<NS> &m, Ctx &cx, double t0,
       // directions of all stages in one pass after the loop
       double ga = 0.0;
 #pragma unroll
-      for (int q = 0; q < FT_N; ++q) ga += dfa[q] * fb[q];
+      for (int q = 0; q < FT_N; ++q) ga += (double)dfa[q] * fb[q];
       if (lane == L_A) ub[0] += ga;
       cst[i] = ai;
 #pragma unroll
