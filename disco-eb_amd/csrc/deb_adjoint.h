@@ -644,8 +644,9 @@ __device__ __forceinline__ double lane_dot(const TL (&F)[FT_N], const double (&f
   return g;
 }
 
-// feature values at a (plain doubles)
-DEB_OUTLINE void feature_values(Ctx &cx, double a, double (&fv)[FT_N]) {
+// feature values at a (plain doubles), wave-uniform: stored once per wave in
+// LDS (every lane writes the same values)
+DEB_OUTLINE void feature_values(Ctx &cx, double a, lds_double *fv) {
   double f[FT_N];
   features_t<double>(cx, a, f);
 #pragma unroll
@@ -748,10 +749,10 @@ DEB_OUTLINE void acol_pass(Ctx &cx, const uint32_t (&rd)[NS], dual a, double inv
 // Second order along one direction per lane: with P_m = dS/dF_m and
 // Q_m = dP_m/da (wave-uniform) for S(F(a, theta)), returns on lane direction c
 // d/dc (dS/da) = sum_m Q_m dF_m/dc + P_m d2F_m/(da dc)  (c = a on lane & 7 == 0).
-// Features in nested duals: outer component along the lane's direction, inner
-// along a.
+// PQ: P (FT_N doubles), then Q, one copy per wave in LDS.  Features in nested
+// duals: outer component along the lane's direction, inner along a.
 using TL2 = dn<2, dual>;
-DEB_OUTLINE double lane_contract2(const Ctx &cx, double a, const double (&P)[FT_N], const double (&Q)[FT_N]) {
+DEB_OUTLINE double lane_contract2(const Ctx &cx, double a, const lds_double *PQ) {
   Ctx cl = lane_ctx(cx);
   TL2 as;
   as = a;
@@ -762,7 +763,7 @@ DEB_OUTLINE double lane_contract2(const Ctx &cx, double a, const double (&P)[FT_
   double g = 0.0;
 #pragma unroll
   for (int m = 0; m < FT_N; ++m)
-    g += Q[m] * (F[m].d[0].v + F[m].d[1].v) + P[m] * (F[m].d[0].d + F[m].d[1].d);
+    g += (double)PQ[FT_N + m] * (F[m].d[0].v + F[m].d[1].v) + (double)PQ[m] * (F[m].d[0].d + F[m].d[1].d);
   return g;
 }
 
@@ -924,12 +925,7 @@ __device__ __forceinline__ void rodas5_step_vjp(Mode<NS> &m, Ctx &cx, double t0,
   __shared__ double fvJ_[FT_N], dfa_[FT_N];
   lds_double *const fvJ = (lds_double *)fvJ_, *const dfa = (lds_double *)dfa_;
   double fbJs[FT_N];
-  {
-    double f[FT_N];
-    feature_values(cx, a0, f);
-#pragma unroll
-    for (int q = 0; q < FT_N; ++q) fvJ[q] = f[q];
-  }
+  feature_values(cx, a0, fvJ);
 #pragma unroll
   for (int q = 0; q < FT_N; ++q) fbJs[q] = 0.0;
 #pragma nounroll
@@ -1079,8 +1075,9 @@ __device__ __forceinline__ void rodas5_step_vjp(Mode<NS> &m, Ctx &cx, double t0,
     acol_pass<NS>(cx, m.rd, ad, it0, rho, y0, gy, fbd);
 #pragma unroll
     for (int s = 0; s < NS; ++s) y0b[s] += gy[s].d;
-    // the feature partials P and their a-derivative Q, summed over the wave,
-    // then the features' second derivatives
+    // the feature partials P and their a-derivative Q, summed over the wave
+    // and kept in the stage block (whose contraction has been read), then the
+    // features' second derivatives
     double PQ[2 * FT_N];
 #pragma unroll
     for (int q = 0; q < FT_N; ++q) {
@@ -1088,13 +1085,12 @@ __device__ __forceinline__ void rodas5_step_vjp(Mode<NS> &m, Ctx &cx, double t0,
       PQ[FT_N + q] = fbd[q].d;
     }
     wave_sum<2 * FT_N>(PQ);
-    double P[FT_N], Q[FT_N];
 #pragma unroll
     for (int q = 0; q < FT_N; ++q) {
-      P[q] = PQ[q];
-      Q[q] = PQ[FT_N + q] + fbJ[q];
+      cst[q] = PQ[q];
+      cst[FT_N + q] = PQ[FT_N + q] + fbJ[q];
     }
-    const double g = lane_contract2(cx, a0, P, Q);
+    const double g = lane_contract2(cx, a0, cst);
     a0b += readlane(g, 0);
     double tl[MAXDIR];
     lane_gather(g, cx.ndir, tl);
