@@ -794,13 +794,14 @@ __device__ __forceinline__ RowSums jac_sums(const Mode<NS> &m, const double (&Y)
 // The row work of one reverse stage as a single call: the transposed row pass
 // at the stage point (ub = J(t_i, u_i)^T rb, the stage's features and their
 // a-derivative dfa), the row sums of the J(t0, y0) k_i term, and the feature
-// partials of both contractions -- the stage's into fb0, the J-term's (at the
-// features fvJ of a0, multipliers without the a-row) accumulated into fbJ.
+// partials of both contractions -- the J-term's (at the features fvJ of a0,
+// multipliers without the a-row) accumulated per lane into fbJ, the stage's
+// summed over the wave into fb_out (one copy per wave in LDS) and contracted
+// with dfa into the a-direction of ub.
 template <int NS>
 DEB_OUTLINE void stage_rows(Ctx &cx, const Mode<NS> &m, double ai, double iti, const double (&rb_in)[NS],
                             const double (&u_in)[NS], const double (&ki_in)[NS], const lds_double *fvJ_in,
-                            double (&ub_out)[NS], double (&fb0_out)[FT_N], double (&fbJ_io)[FT_N],
-                            lds_double *dfa_out) {
+                            double (&ub_out)[NS], double (&fbJ_io)[FT_N], lds_double *fb_out) {
   const int lane = lane_id();
   double rb[NS], u[NS], ki[NS], ub[NS], dfa[FT_N];
   double fvs[2][FT_N], fbs[2][FT_N];
@@ -828,13 +829,17 @@ DEB_OUTLINE void stage_rows(Ctx &cx, const Mode<NS> &m, double ai, double iti, c
   }
   rsq[1] = jac_sums<NS>(m, ki, wq[1]);
   feature_partials_body<NS, 2>(cx, m.rd, fvs, Yq, wq, rsq, fbs);
+  wave_sum<FT_N>(fbs[0]);
+  double ga = 0.0;
+#pragma unroll
+  for (int q = 0; q < FT_N; ++q) ga += dfa[q] * fbs[0][q];
+  if (lane == L_A) ub[0] += ga;
 #pragma unroll
   for (int s = 0; s < NS; ++s) ub_out[s] = ub[s];
 #pragma unroll
   for (int q = 0; q < FT_N; ++q) {
-    fb0_out[q] = fbs[0][q];
+    fb_out[q] = fbs[0][q];
     fbJ_io[q] = fbs[1][q];
-    dfa_out[q] = dfa[q];
   }
 }
 
@@ -920,8 +925,8 @@ __device__ __forceinline__ void rodas5_step_vjp(Mode<NS> &m, Ctx &cx, double t0,
   // contractions: f(t_i, u_i) at the stage's features (contracted every
   // stage) and the J(t0, y0) k_i term at the features fvJ of a0 (its partials
   // fbJs accumulate over the stages and are contracted once).
-  // the features of a0 and each stage's feature a-derivatives are wave-uniform:
-  // one copy per wave in LDS (written by every lane alike)
+  // the features of a0 and (five slots) each stage's feature a-derivatives are
+  // wave-uniform: one copy per wave in LDS (written by every lane alike)
   __shared__ double fvJ_[FT_N], dfa_[FT_N];
   lds_double *const fvJ = (lds_double *)fvJ_, *const dfa = (lds_double *)dfa_;
   double fbJs[FT_N];
@@ -990,9 +995,8 @@ __device__ __forceinline__ void rodas5_step_vjp(Mode<NS> &m, Ctx &cx, double t0,
     double ub[NS];
     {
       const double ai = readlane(u[0], L_A);
-      double fb[FT_N];
       if constexpr (NS <= 4) {
-        stage_rows<NS>(cx, m, ai, iti, rb, u, ki, fvJ, ub, fb, fbJs, dfa);
+        stage_rows<NS>(cx, m, ai, iti, rb, u, ki, fvJ, ub, fbJs, cst + 8 + i * FT_N);
       } else {
         // five slots: the same work as separate calls (as one call the
         // reverse launch of the lmax-50 workload measured 14 % slower)
@@ -1018,21 +1022,21 @@ __device__ __forceinline__ void rodas5_step_vjp(Mode<NS> &m, Ctx &cx, double t0,
         }
         feature_partials<NS, 2>(cx, m.rd, fvs, Yq, wq, rsq, fbs);
 #pragma unroll
-        for (int q = 0; q < FT_N; ++q) {
-          fb[q] = fbs[0][q];
-          fbJs[q] = fbs[1][q];
-        }
+        for (int q = 0; q < FT_N; ++q) fbJs[q] = fbs[1][q];
+        double fb[FT_N];
+#pragma unroll
+        for (int q = 0; q < FT_N; ++q) fb[q] = fbs[0][q];
+        wave_sum<FT_N>(fb);
+        // the a-direction now (it feeds the earlier stages); the input-tangent
+        // directions of all stages in one pass after the loop
+        double ga = 0.0;
+#pragma unroll
+        for (int q = 0; q < FT_N; ++q) ga += (double)dfa[q] * fb[q];
+        if (lane == L_A) ub[0] += ga;
+#pragma unroll
+        for (int q = 0; q < FT_N; ++q) cst[8 + i * FT_N + q] = fb[q];
       }
-      wave_sum<FT_N>(fb);
-      // the a-direction now (it feeds the earlier stages); the input-tangent
-      // directions of all stages in one pass after the loop
-      double ga = 0.0;
-#pragma unroll
-      for (int q = 0; q < FT_N; ++q) ga += (double)dfa[q] * fb[q];
-      if (lane == L_A) ub[0] += ga;
       cst[i] = ai;
-#pragma unroll
-      for (int q = 0; q < FT_N; ++q) cst[8 + i * FT_N + q] = fb[q];
       const double kia = readlane(ki[0], L_A);
 #pragma unroll
       for (int s = 0; s < NS; ++s) rho[s] += kia * rb[s];
