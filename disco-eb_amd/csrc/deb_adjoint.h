@@ -794,14 +794,14 @@ __device__ __forceinline__ RowSums jac_sums(const Mode<NS> &m, const double (&Y)
 // The row work of one reverse stage as a single call: the transposed row pass
 // at the stage point (ub = J(t_i, u_i)^T rb, the stage's features and their
 // a-derivative dfa), the row sums of the J(t0, y0) k_i term, and the feature
-// partials of both contractions -- the J-term's (at the features fvJ of a0,
-// multipliers without the a-row) accumulated per lane into fbJ, the stage's
-// summed over the wave into fb_out (one copy per wave in LDS) and contracted
-// with dfa into the a-direction of ub.
+// partials of both contractions, each summed over the wave: the J-term's (at
+// the features fvJ of a0, multipliers without the a-row) accumulated into fbJ,
+// the stage's stored to fb_out and contracted with dfa into the a-direction of
+// ub (fbJ and fb_out: one copy per wave in LDS).
 template <int NS>
 DEB_OUTLINE void stage_rows(Ctx &cx, const Mode<NS> &m, double ai, double iti, const double (&rb_in)[NS],
                             const double (&u_in)[NS], const double (&ki_in)[NS], const lds_double *fvJ_in,
-                            double (&ub_out)[NS], double (&fbJ_io)[FT_N], lds_double *fb_out) {
+                            double (&ub_out)[NS], lds_double *fbJ_io, lds_double *fb_out) {
   const int lane = lane_id();
   double rb[NS], u[NS], ki[NS], ub[NS], dfa[FT_N];
   double fvs[2][FT_N], fbs[2][FT_N];
@@ -815,7 +815,7 @@ DEB_OUTLINE void stage_rows(Ctx &cx, const Mode<NS> &m, double ai, double iti, c
   for (int q = 0; q < FT_N; ++q) {
     fvs[1][q] = (double)fvJ_in[q];
     fbs[0][q] = 0.0;
-    fbs[1][q] = fbJ_io[q];
+    fbs[1][q] = 0.0;
   }
   RowSums rsq[2];
   rsq[0] = coef_pass_body<NS, true, true>(cx, m.rd, ai, iti, rb, u, ub, fvs[0], dfa);
@@ -830,6 +830,7 @@ DEB_OUTLINE void stage_rows(Ctx &cx, const Mode<NS> &m, double ai, double iti, c
   rsq[1] = jac_sums<NS>(m, ki, wq[1]);
   feature_partials_body<NS, 2>(cx, m.rd, fvs, Yq, wq, rsq, fbs);
   wave_sum<FT_N>(fbs[0]);
+  wave_sum<FT_N>(fbs[1]);
   double ga = 0.0;
 #pragma unroll
   for (int q = 0; q < FT_N; ++q) ga += dfa[q] * fbs[0][q];
@@ -839,7 +840,7 @@ DEB_OUTLINE void stage_rows(Ctx &cx, const Mode<NS> &m, double ai, double iti, c
 #pragma unroll
   for (int q = 0; q < FT_N; ++q) {
     fb_out[q] = fbs[0][q];
-    fbJ_io[q] = fbs[1][q];
+    fbJ_io[q] += fbs[1][q];
   }
 }
 
@@ -925,14 +926,18 @@ __device__ __forceinline__ void rodas5_step_vjp(Mode<NS> &m, Ctx &cx, double t0,
   // contractions: f(t_i, u_i) at the stage's features (contracted every
   // stage) and the J(t0, y0) k_i term at the features fvJ of a0 (its partials
   // fbJs accumulate over the stages and are contracted once).
-  // the features of a0 and (five slots) each stage's feature a-derivatives are
-  // wave-uniform: one copy per wave in LDS (written by every lane alike)
+  // the features of a0 and each stage's feature a-derivatives (five slots) or
+  // the J-term's summed partials (four slots or fewer) are wave-uniform: one
+  // copy per wave in LDS (written by every lane alike)
   __shared__ double fvJ_[FT_N], dfa_[FT_N];
-  lds_double *const fvJ = (lds_double *)fvJ_, *const dfa = (lds_double *)dfa_;
+  lds_double *const fvJ = (lds_double *)fvJ_, *const dfa = (lds_double *)dfa_, *const fbJl = (lds_double *)dfa_;
   double fbJs[FT_N];
   feature_values(cx, a0, fvJ);
 #pragma unroll
-  for (int q = 0; q < FT_N; ++q) fbJs[q] = 0.0;
+  for (int q = 0; q < FT_N; ++q) {
+    fbJs[q] = 0.0;
+    if constexpr (NS <= 4) fbJl[q] = 0.0;
+  }
 #pragma nounroll
   for (int i = 7; i >= 0; --i) {
     double rb[NS], ki[NS];
@@ -996,7 +1001,7 @@ __device__ __forceinline__ void rodas5_step_vjp(Mode<NS> &m, Ctx &cx, double t0,
     {
       const double ai = readlane(u[0], L_A);
       if constexpr (NS <= 4) {
-        stage_rows<NS>(cx, m, ai, iti, rb, u, ki, fvJ, ub, fbJs, cst + 8 + i * FT_N);
+        stage_rows<NS>(cx, m, ai, iti, rb, u, ki, fvJ, ub, fbJl, cst + 8 + i * FT_N);
       } else {
         // five slots: the same work as separate calls (as one call the
         // reverse launch of the lmax-50 workload measured 14 % slower)
@@ -1065,9 +1070,14 @@ __device__ __forceinline__ void rodas5_step_vjp(Mode<NS> &m, Ctx &cx, double t0,
   // the J(t0, y0) k_i terms' partials, contracted with the feature tangents
   // at a0 together with the a-column's below (first order: added to Q)
   double fbJ[FT_N];
+  if constexpr (NS <= 4) {
 #pragma unroll
-  for (int q = 0; q < FT_N; ++q) fbJ[q] = fbJs[q];
-  wave_sum<FT_N>(fbJ);
+    for (int q = 0; q < FT_N; ++q) fbJ[q] = fbJl[q];
+  } else {
+#pragma unroll
+    for (int q = 0; q < FT_N; ++q) fbJ[q] = fbJs[q];
+    wave_sum<FT_N>(fbJ);
+  }
   // a-column of J(t0, y0): rho^T df/da, differentiated along y0, a0 and theta
   {
     dual ad;
