@@ -612,6 +612,39 @@ DEB_OUTLINE void feature_partials(const Ctx &cx, const uint32_t (&rd)[NS], const
   feature_partials_body<NS, K, T>(cx, rd, fv_in, Y_in, w_in, rs_in, fb);
 }
 
+// The feature partials of one reverse stage at five slots (after its row pass
+// and row sums): both contractions summed over the wave, the stage's (at its
+// features fv) stored to fb_out and contracted with the features'
+// a-derivatives dfa into the returned a-direction of the stage cotangent, the
+// J-term's (at the features fvJ of a0) accumulated into fbJ.  fvJ, fbJ and
+// fb_out: one copy per wave in LDS.
+template <int NS>
+DEB_OUTLINE double stage_partials(const Ctx &cx, const uint32_t (&rd)[NS], const double (&fv_in)[FT_N],
+                                  const lds_double *fvJ_in, const double (&dfa_in)[FT_N],
+                                  const double (&Y_in)[2][NS], const double (&w_in)[2][NS],
+                                  const RowSums (&rs_in)[2], lds_double *fbJ_io, lds_double *fb_out) {
+  double fvs[2][FT_N], fbs[2][FT_N];
+#pragma unroll
+  for (int q = 0; q < FT_N; ++q) {
+    fvs[0][q] = fv_in[q];
+    fvs[1][q] = (double)fvJ_in[q];
+    fbs[0][q] = 0.0;
+    fbs[1][q] = 0.0;
+  }
+  feature_partials_body<NS, 2>(cx, rd, fvs, Y_in, w_in, rs_in, fbs);
+  wave_sum<FT_N>(fbs[0]);
+  wave_sum<FT_N>(fbs[1]);
+  double ga = 0.0;
+#pragma unroll
+  for (int q = 0; q < FT_N; ++q) ga += dfa_in[q] * fbs[0][q];
+#pragma unroll
+  for (int q = 0; q < FT_N; ++q) {
+    fb_out[q] = fbs[0][q];
+    fbJ_io[q] += fbs[1][q];
+  }
+  return ga;
+}
+
 // One tangent direction per lane: lane & 7 == 0 carries d/da (component 0 of
 // TL, seeded by seed_tl), lane & 7 == c (1 <= c <= ndir) the input tangent
 // c - 1 (component 1, loaded through lane_ctx); TL arithmetic then costs
@@ -925,19 +958,14 @@ __device__ __forceinline__ void rodas5_step_vjp(Mode<NS> &m, Ctx &cx, double t0,
   // Feature partials, one row pass per stage (stage_rows) with two
   // contractions: f(t_i, u_i) at the stage's features (contracted every
   // stage) and the J(t0, y0) k_i term at the features fvJ of a0 (its partials
-  // fbJs accumulate over the stages and are contracted once).
-  // the features of a0 and each stage's feature a-derivatives (five slots) or
-  // the J-term's summed partials (four slots or fewer) are wave-uniform: one
-  // copy per wave in LDS (written by every lane alike)
-  __shared__ double fvJ_[FT_N], dfa_[FT_N];
-  lds_double *const fvJ = (lds_double *)fvJ_, *const dfa = (lds_double *)dfa_, *const fbJl = (lds_double *)dfa_;
-  double fbJs[FT_N];
+  // fbJ accumulate over the stages and are contracted once).
+  // the features of a0 and the J-term's partials (summed over the wave) are
+  // wave-uniform: one copy per wave in LDS (written by every lane alike)
+  __shared__ double fvJ_[FT_N], fbJ_[FT_N];
+  lds_double *const fvJ = (lds_double *)fvJ_, *const fbJl = (lds_double *)fbJ_;
   feature_values(cx, a0, fvJ);
 #pragma unroll
-  for (int q = 0; q < FT_N; ++q) {
-    fbJs[q] = 0.0;
-    if constexpr (NS <= 4) fbJl[q] = 0.0;
-  }
+  for (int q = 0; q < FT_N; ++q) fbJl[q] = 0.0;
 #pragma nounroll
   for (int i = 7; i >= 0; --i) {
     double rb[NS], ki[NS];
@@ -1005,12 +1033,9 @@ __device__ __forceinline__ void rodas5_step_vjp(Mode<NS> &m, Ctx &cx, double t0,
       } else {
         // five slots: the same work as separate calls (as one call the
         // reverse launch of the lmax-50 workload measured 14 % slower)
-        double fvs[2][FT_N], fbs[2][FT_N], Yq[2][NS], wq[2][NS];
+        double fvs[FT_N], dfl[FT_N], Yq[2][NS], wq[2][NS];
         RowSums rsq[2];
-        double dfl[FT_N];
-        rsq[0] = coef_pass<NS, true, true>(cx, m.rd, ai, iti, rb, u, ub, fvs[0], dfl);
-#pragma unroll
-        for (int q = 0; q < FT_N; ++q) dfa[q] = dfl[q];
+        rsq[0] = coef_pass<NS, true, true>(cx, m.rd, ai, iti, rb, u, ub, fvs, dfl);
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
           Yq[0][s] = u[s];
@@ -1019,27 +1044,10 @@ __device__ __forceinline__ void rodas5_step_vjp(Mode<NS> &m, Ctx &cx, double t0,
           wq[1][s] = (s == 0 && lane == L_A) ? 0.0 : rb[s];
         }
         rsq[1] = jac_sums<NS>(m, ki, wq[1]);
-#pragma unroll
-        for (int q = 0; q < FT_N; ++q) {
-          fvs[1][q] = (double)fvJ[q];
-          fbs[0][q] = 0.0;
-          fbs[1][q] = fbJs[q];
-        }
-        feature_partials<NS, 2>(cx, m.rd, fvs, Yq, wq, rsq, fbs);
-#pragma unroll
-        for (int q = 0; q < FT_N; ++q) fbJs[q] = fbs[1][q];
-        double fb[FT_N];
-#pragma unroll
-        for (int q = 0; q < FT_N; ++q) fb[q] = fbs[0][q];
-        wave_sum<FT_N>(fb);
         // the a-direction now (it feeds the earlier stages); the input-tangent
         // directions of all stages in one pass after the loop
-        double ga = 0.0;
-#pragma unroll
-        for (int q = 0; q < FT_N; ++q) ga += (double)dfa[q] * fb[q];
+        const double ga = stage_partials<NS>(cx, m.rd, fvs, fvJ, dfl, Yq, wq, rsq, fbJl, cst + 8 + i * FT_N);
         if (lane == L_A) ub[0] += ga;
-#pragma unroll
-        for (int q = 0; q < FT_N; ++q) cst[8 + i * FT_N + q] = fb[q];
       }
       cst[i] = ai;
       const double kia = readlane(ki[0], L_A);
@@ -1070,14 +1078,8 @@ __device__ __forceinline__ void rodas5_step_vjp(Mode<NS> &m, Ctx &cx, double t0,
   // the J(t0, y0) k_i terms' partials, contracted with the feature tangents
   // at a0 together with the a-column's below (first order: added to Q)
   double fbJ[FT_N];
-  if constexpr (NS <= 4) {
 #pragma unroll
-    for (int q = 0; q < FT_N; ++q) fbJ[q] = fbJl[q];
-  } else {
-#pragma unroll
-    for (int q = 0; q < FT_N; ++q) fbJ[q] = fbJs[q];
-    wave_sum<FT_N>(fbJ);
-  }
+  for (int q = 0; q < FT_N; ++q) fbJ[q] = fbJl[q];
   // a-column of J(t0, y0): rho^T df/da, differentiated along y0, a0 and theta
   {
     dual ad;
